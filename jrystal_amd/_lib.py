@@ -1,0 +1,114 @@
+"""ctypes binding of libjrx_hip.so (the HIP implementation, C ABI in
+include/jrx.h).  There is no CPU fallback: importing a GPU entry point
+without the built library raises."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjrx_hip.so")
+
+JRX_OK = 0
+JRX_ERR_INVALID = 1
+JRX_ERR_UNSUPPORTED = 2
+JRX_ERR_HIP = 3
+JRX_ERR_NOMEM = 4
+
+_lib = None
+
+
+class JrxProblem(ctypes.Structure):
+  _fields_ = [
+    ("ns", ctypes.c_int), ("nk", ctypes.c_int), ("nb", ctypes.c_int),
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+    ("natom", ctypes.c_int),
+    ("mask", ctypes.c_void_p), ("kpts", ctypes.c_void_p),
+    ("cell", ctypes.c_void_p), ("positions", ctypes.c_void_p),
+    ("charges", ctypes.c_void_p), ("occupation", ctypes.c_void_p),
+    ("xc", ctypes.c_int), ("chunk_states", ctypes.c_int),
+  ]
+
+
+class JrxPlanInfo(ctypes.Structure):
+  _fields_ = [("ng", ctypes.c_int), ("active_planes", ctypes.c_int),
+              ("chunk_states", ctypes.c_int),
+              ("workspace_bytes", ctypes.c_size_t)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+# name -> argtypes (restype is int unless listed in _RESTYPES)
+SIGNATURES = {
+  "jrx_last_error": [],
+  "jrx_version": [],
+  "jrx_fftn3d": [_P, _P, _I64, _I, _I, _I, _D, _P],
+  "jrx_ifftn3d": [_P, _P, _I64, _I, _I, _I, _D, _P],
+  "jrx_plan_create": [ctypes.POINTER(ctypes.c_void_p),
+                      ctypes.POINTER(JrxProblem)],
+  "jrx_plan_destroy": [_P],
+  "jrx_plan_workspace_bytes": [_P, ctypes.POINTER(_SZ)],
+  "jrx_plan_get_info": [_P, ctypes.POINTER(JrxPlanInfo)],
+  "jrx_energy_grad": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_set_timing": [_P, _I],
+  "jrx_plan_read_timing": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
+}
+
+# Kernel classes of jrx_plan_read_timing (enum JRX_K_* in include/jrx.h).
+KERNEL_CLASSES = (
+  "qr_gram", "qr_small", "qr_apply", "plane_ifft", "xline_density",
+  "potential", "xline_vpsi", "plane_fft_gather", "qr_bwd_gram",
+  "qr_bwd_apply", "other",
+)
+_RESTYPES = {"jrx_last_error": ctypes.c_char_p,
+             "jrx_version": ctypes.c_char_p}
+
+
+def lib():
+  """Load the library (once).  Raises RuntimeError when it is not built."""
+  global _lib
+  if _lib is not None:
+    return _lib
+  if not os.path.exists(LIB_PATH):
+    raise RuntimeError(
+      f"{LIB_PATH} is missing: build it with `python -m jrystal_amd.build` "
+      "(the HIP path has no CPU fallback)")
+  h = ctypes.CDLL(LIB_PATH)
+  for name, args in SIGNATURES.items():
+    fn = getattr(h, name)
+    fn.argtypes = args
+    fn.restype = _RESTYPES.get(name, ctypes.c_int)
+  _lib = h
+  return h
+
+
+def check(rc):
+  """Map a jrx status code to a Python exception."""
+  if rc == JRX_OK:
+    return
+  msg = lib().jrx_last_error().decode()
+  if rc in (JRX_ERR_INVALID,):
+    raise ValueError(msg)
+  if rc == JRX_ERR_UNSUPPORTED:
+    raise NotImplementedError(msg)
+  if rc == JRX_ERR_NOMEM:
+    raise MemoryError(msg)
+  raise RuntimeError(msg)
+
+
+def ptr(t):
+  """Device (or host numpy) pointer of a tensor/array, None -> NULL."""
+  if t is None:
+    return None
+  if isinstance(t, np.ndarray):
+    return t.ctypes.data
+  return t.data_ptr()
+
+
+def stream_ptr(device=None):
+  import torch
+  return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,230 @@
+// Reciprocal-space Hartree / external energies, LDA exchange, kinetic
+// energy and the potential v(r) = dE/drho(r) for the adjoint.
+//
+// Formulas follow jrystal/_src/potential.py:24-77,121-166 (note the literal
+// +1e-10 regulariser of the external term), braket.py:27-72,
+// energy.py:29-211 and xc.py:54-64.
+#include "energy_kernels.h"
+
+namespace jrx {
+
+__device__ __forceinline__ int freq(int j, int n) {
+  return j < (n + 1) / 2 ? j : j - n;
+}
+
+__global__ __launch_bounds__(kRedBlock) void rho_to_complex_kernel(
+    const double* __restrict__ rho, double2* __restrict__ out, int ns,
+    int64_t N) {
+  const int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
+  if (i >= N) return;
+  double r = 0.0;
+  for (int s = 0; s < ns; ++s) r += rho[s * N + i];
+  out[i] = make_double2(r, 0.0);
+}
+
+int launch_rho_to_complex(const double* rho, double2* out, int ns, int64_t N,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(rho_to_complex_kernel, dim3(ceil_div(N, kRedBlock)),
+                     dim3(kRedBlock), 0, s, rho, out, ns, N);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+__global__ __launch_bounds__(kRedBlock) void potential_rec_kernel(
+    const double2* __restrict__ rho_rec, double2* __restrict__ vrec,
+    double* __restrict__ eh_part, double* __restrict__ ee_part,
+    CellParams cp) {
+  __shared__ double red[kRedBlock];
+  const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
+  const int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
+  double eh = 0.0, ee = 0.0;
+  if (i < N) {
+    const int iz = (int)(i % cp.nz);
+    const int iy = (int)((i / cp.nz) % cp.ny);
+    const int ix = (int)(i / ((int64_t)cp.ny * cp.nz));
+    const int mx = freq(ix, cp.nx), my = freq(iy, cp.ny), mz = freq(iz, cp.nz);
+    const double gx = mx * cp.B[0] + my * cp.B[3] + mz * cp.B[6];
+    const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7];
+    const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8];
+    const double g2 = gx * gx + gy * gy + gz * gz;
+    double2 out = czero();
+    if (i != 0) {
+      double sr = 0.0, si = 0.0;  // S(G) = sum_a Z_a e^{+i G.R_a}
+      for (int a = 0; a < cp.natom; ++a) {
+        const double ph = gx * cp.pos[3 * a] + gy * cp.pos[3 * a + 1] +
+                          gz * cp.pos[3 * a + 2];
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        sr += cp.charge[a] * cs;
+        si += cp.charge[a] * sn;
+      }
+      const double2 r = rho_rec[i];
+      const double inv_g2 = 1.0 / g2;
+      const double inv_g2e = 1.0 / (g2 + 1e-10);
+      const double dN = (double)N;
+      eh = (2.0 * M_PI * cp.vol / (dN * dN)) * (r.x * r.x + r.y * r.y) * inv_g2;
+      ee = -(4.0 * M_PI / dN) * (sr * r.x - si * r.y) * inv_g2e;
+      const double ch = 4.0 * M_PI * cp.vol / dN * inv_g2;
+      const double ce = -4.0 * M_PI * inv_g2e;
+      out = make_double2(ch * r.x + ce * sr, ch * r.y - ce * si);
+    }
+    vrec[i] = out;
+  }
+  const double teh = block_sum(eh, red);
+  const double tee = block_sum(ee, red);
+  if (threadIdx.x == 0) {
+    eh_part[blockIdx.x] = teh;
+    ee_part[blockIdx.x] = tee;
+  }
+}
+
+int launch_potential_rec(const double2* rho_rec, double2* vrec,
+                         double* eh_part, double* ee_part,
+                         const CellParams& cp, int* nparts, hipStream_t s) {
+  const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
+  const int nb = ceil_div(N, kRedBlock);
+  hipLaunchKernelGGL(potential_rec_kernel, dim3(nb), dim3(kRedBlock), 0, s,
+                     rho_rec, vrec, eh_part, ee_part, cp);
+  JRX_LAUNCH_CHECK();
+  *nparts = nb;
+  return JRX_OK;
+}
+
+// eps_x(n) = -cx n^(1/3) above the density threshold, else 0.
+__device__ __forceinline__ double lda_x_eps(double n, double cx) {
+  return n > 1e-15 ? -cx * cbrt(n) : 0.0;
+}
+__device__ __forceinline__ double lda_x_deps(double n, double cx) {
+  if (!(n > 1e-15)) return 0.0;
+  const double c = cbrt(n);
+  return -cx / (3.0 * c * c);
+}
+
+__global__ __launch_bounds__(kRedBlock) void potential_real_kernel(
+    const double2* __restrict__ vbox, const double* __restrict__ rho,
+    double* __restrict__ v, double* __restrict__ exc_part, int ns, int64_t N,
+    double vol, double cx) {
+  __shared__ double red[kRedBlock];
+  const int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
+  double exc = 0.0;
+  if (i < N) {
+    const double w = vol / (double)N;
+    const double vhe = vbox[i].x;
+    if (ns == 1) {
+      const double r = rho[i];
+      const double e = lda_x_eps(r, cx);
+      exc = w * e * r;
+      v[i] = vhe + w * (4.0 / 3.0) * e;
+    } else {
+      // Spin-polarised exchange as in xc.py:56-59: eps = (e(2ru)+e(2rd))/2
+      // multiplies the total density.
+      const double ru = rho[i], rd = rho[N + i];
+      const double e = 0.5 * (lda_x_eps(2.0 * ru, cx) + lda_x_eps(2.0 * rd, cx));
+      const double rt = ru + rd;
+      exc = w * e * rt;
+      v[i] = vhe + w * (e + rt * lda_x_deps(2.0 * ru, cx));
+      v[N + i] = vhe + w * (e + rt * lda_x_deps(2.0 * rd, cx));
+    }
+  }
+  const double t = block_sum(exc, red);
+  if (threadIdx.x == 0) exc_part[blockIdx.x] = t;
+}
+
+int launch_potential_real(const double2* vbox, const double* rho, double* v,
+                          double* exc_part, int ns, int64_t N, double vol,
+                          double cx, int* nparts, hipStream_t s) {
+  const int nb = ceil_div(N, kRedBlock);
+  hipLaunchKernelGGL(potential_real_kernel, dim3(nb), dim3(kRedBlock), 0, s,
+                     vbox, rho, v, exc_part, ns, N, vol, cx);
+  JRX_LAUNCH_CHECK();
+  *nparts = nb;
+  return JRX_OK;
+}
+
+// g_m[3g..3g+2] are the integer frequencies (mx, my, mz) of mask point g.
+__global__ __launch_bounds__(kRedBlock) void kinetic_kernel(
+    const double2* __restrict__ q, const double* __restrict__ occ,
+    const double* __restrict__ kpts, const short* __restrict__ g_m,
+    CellParams cp, int ng, int nk, int nb, int gblocks,
+    double* __restrict__ kin_part) {
+  __shared__ double red[kRedBlock];
+  const int64_t st = blockIdx.x / gblocks;
+  const int gbk = (int)(blockIdx.x % gblocks);
+  const int g = gbk * kRedBlock + threadIdx.x;
+  double kin = 0.0;
+  if (g < ng) {
+    const int k = (int)((st / nb) % nk);
+    const int mx = g_m[3 * g], my = g_m[3 * g + 1], mz = g_m[3 * g + 2];
+    const double gx = mx * cp.B[0] + my * cp.B[3] + mz * cp.B[6] + kpts[3 * k];
+    const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7] + kpts[3 * k + 1];
+    const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8] + kpts[3 * k + 2];
+    const double2 c = q[st * ng + g];
+    kin = occ[st] * (gx * gx + gy * gy + gz * gz) * (c.x * c.x + c.y * c.y);
+  }
+  const double t = block_sum(kin, red);
+  if (threadIdx.x == 0) kin_part[st * gblocks + gbk] = t;
+}
+
+int launch_kinetic(const double2* q, const double* occ, const double* kpts,
+                   const short* g_m, const CellParams& cp, int64_t nstates,
+                   int ng, int nk, int nb, double* kin_part, int* nparts,
+                   hipStream_t s) {
+  const int gb = ceil_div(ng, kRedBlock);
+  if ((int64_t)gb * nstates > 0x7fffffff) {
+    set_error("too many states for the kinetic reduction");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(kinetic_kernel, dim3((unsigned)(gb * nstates)),
+                     dim3(kRedBlock), 0, s, q, occ, kpts, g_m, cp, ng, nk, nb,
+                     gb, kin_part);
+  JRX_LAUNCH_CHECK();
+  *nparts = (int)(gb * nstates);
+  return JRX_OK;
+}
+
+__device__ double strided_sum(const double* p, int n, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRedBlock) acc += p[i];
+  return block_sum(acc, red);
+}
+
+__global__ __launch_bounds__(kRedBlock) void finalize_energies_kernel(
+    const double* kin, int nkin, const double* eh, const double* ee, int nrec,
+    const double* exc, int nexc, double* energies) {
+  __shared__ double red[kRedBlock];
+  const double k = 0.5 * strided_sum(kin, nkin, red);
+  const double h = strided_sum(eh, nrec, red);
+  const double e = strided_sum(ee, nrec, red);
+  const double x = strided_sum(exc, nexc, red);
+  if (threadIdx.x == 0) {
+    energies[0] = k;
+    energies[1] = h;
+    energies[2] = e;
+    energies[3] = x;
+    energies[4] = ((k + h) + e) + x;
+  }
+}
+
+int launch_finalize_energies(const double* kin, int nkin, const double* eh,
+                             const double* ee, int nrec, const double* exc,
+                             int nexc, double* energies, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_energies_kernel, dim3(1), dim3(kRedBlock), 0, s,
+                     kin, nkin, eh, ee, nrec, exc, nexc, energies);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+__global__ void fill_kernel(double* p, double v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int launch_fill(double* p, double v, int64_t n, hipStream_t s) {
+  if (n <= 0) return JRX_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, p,
+                     v, n);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+}  // namespace jrx

@@ -1,0 +1,51 @@
+// Host-side launchers of the reciprocal/real-space energy kernels.
+#pragma once
+#include "common.h"
+
+namespace jrx {
+
+constexpr int kRedBlock = 256;
+
+struct CellParams {
+  int nx, ny, nz;
+  double B[9];       // reciprocal basis rows, B = 2 pi A^{-T}
+  double vol;
+  int natom;
+  const double* pos;     // device [natom*3]
+  const double* charge;  // device [natom]
+};
+
+// out[i] = (sum_s rho[s*N + i], 0)
+int launch_rho_to_complex(const double* rho, double2* out, int ns, int64_t N,
+                          hipStream_t s);
+
+// From rho_rec = fftn(rho_tot) build the reciprocal potential
+//   vrec(G) = (4 pi vol / N) rho_rec / |G|^2 - 4 pi conj(S(G)) / (|G|^2+1e-10)
+// (zero at G = 0; S(G) = sum_a Z_a e^{i G.R_a}) and per-block partials of
+//   E_H   = (2 pi vol / N^2) sum_{G!=0} |rho_rec|^2 / |G|^2
+//   E_ext = -(4 pi / N) sum_{G!=0} Re(S rho_rec) / (|G|^2 + 1e-10).
+// vrec may alias rho_rec.  Returns the number of partials written.
+int launch_potential_rec(const double2* rho_rec, double2* vrec,
+                         double* eh_part, double* ee_part,
+                         const CellParams& cp, int* nparts, hipStream_t s);
+
+// v_s(r) = Re(vbox(r)) + v_xc,s(r) and partials of E_xc (LDA exchange).
+int launch_potential_real(const double2* vbox, const double* rho, double* v,
+                          double* exc_part, int ns, int64_t N, double vol,
+                          double cx, int* nparts, hipStream_t s);
+
+// Kinetic partials f |G+k|^2 |q|^2 per (state, g-block), band-major q.
+int launch_kinetic(const double2* q, const double* occ, const double* kpts,
+                   const short* g_m, const CellParams& cp, int64_t nstates,
+                   int ng, int nk, int nb, double* kin_part, int* nparts,
+                   hipStream_t s);
+
+// energies[0..4] = kinetic, hartree, external, xc, total from the partials
+// (fixed summation order).
+int launch_finalize_energies(const double* kin, int nkin, const double* eh,
+                             const double* ee, int nrec, const double* exc,
+                             int nexc, double* energies, hipStream_t s);
+
+int launch_fill(double* p, double v, int64_t n, hipStream_t s);
+
+}  // namespace jrx

@@ -1,0 +1,378 @@
+// FFT kernels: generic batched 3D transforms and the sphere-pruned fused
+// passes of the energy/gradient evaluation.
+#include "fft_kernels.h"
+
+namespace jrx {
+
+// ------------------------------------------------------------ generic 3D --
+constexpr int kLineBlock = 256;
+constexpr int kLineCap = 2048;
+
+// Lines (o, q): base = o*os + q*qs, element stride es.  A block transforms
+// LB consecutive q of one o.  ELEM_MAJOR keeps LDS as [j][LB] (lines
+// adjacent, for strided axes); otherwise [l][n+1] (for contiguous lines).
+template <bool INV, bool ELEM_MAJOR>
+__global__ __launch_bounds__(kLineBlock) void line_fft_kernel(
+    const double2* __restrict__ in, double2* __restrict__ out, FftLen L,
+    int64_t os, int64_t Q, int64_t qs, int64_t es, int LB, double scale) {
+  extern __shared__ double2 sm[];
+  const int n = L.n;
+  const int64_t qblocks = (Q + LB - 1) / LB;
+  const int64_t o = blockIdx.x / qblocks;
+  const int64_t q0 = (blockIdx.x % qblocks) * LB;
+  const int nl = (int)((Q - q0) < LB ? (Q - q0) : LB);
+  const int64_t off = o * os + q0 * qs;
+  const double2* src = in + off;
+  double2* dst = out + off;
+  const int tot = nl * n;
+  for (int e = threadIdx.x; e < tot; e += kLineBlock) {
+    int l, j;
+    if (ELEM_MAJOR) { l = e % nl; j = e / nl; } else { j = e % n; l = e / n; }
+    sm[ELEM_MAJOR ? j * LB + l : l * (n + 1) + j] = src[l * qs + j * es];
+  }
+  __syncthreads();
+  if (ELEM_MAJOR)
+    lds_fft<INV, kLineBlock, kLineCap, true>(sm, nl, 1, LB, nullptr, L);
+  else
+    lds_fft<INV, kLineBlock, kLineCap, false>(sm, nl, n + 1, 1, nullptr, L);
+  for (int e = threadIdx.x; e < tot; e += kLineBlock) {
+    int l, j;
+    if (ELEM_MAJOR) { l = e % nl; j = e / nl; } else { j = e % n; l = e / n; }
+    dst[l * qs + j * es] =
+        cscale(sm[ELEM_MAJOR ? j * LB + l : l * (n + 1) + j], scale);
+  }
+}
+
+int launch_line_fft(const double2* in, double2* out, int64_t B, int n0, int n1,
+                    int n2, int axis, bool inverse, double scale,
+                    const FftLen& L, hipStream_t s) {
+  const int n = L.n;
+  if (n > kLineCap) {
+    set_error("FFT length above 2048 is not supported");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  int LB = kLineCap / n;
+  int64_t nouter, os, Q, qs, es;
+  bool elem_major;
+  if (axis == 2) {
+    nouter = 1; os = 0; Q = B * n0 * n1; qs = n2; es = 1; elem_major = false;
+  } else if (axis == 1) {
+    nouter = B * n0; os = (int64_t)n1 * n2; Q = n2; qs = 1; es = n2;
+    elem_major = true;
+  } else {
+    nouter = B; os = (int64_t)n0 * n1 * n2; Q = (int64_t)n1 * n2; qs = 1;
+    es = (int64_t)n1 * n2; elem_major = true;
+  }
+  if (Q < LB) LB = (int)Q;
+  const int64_t qblocks = (Q + LB - 1) / LB;
+  const int64_t nblocks = nouter * qblocks;
+  if (nblocks <= 0) return JRX_OK;
+  if (nblocks > 0x7fffffff) {
+    set_error("FFT batch too large");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  size_t shm = elem_major ? (size_t)n * LB * sizeof(double2)
+                          : (size_t)LB * (n + 1) * sizeof(double2);
+  dim3 grid((unsigned)nblocks), block(kLineBlock);
+  if (inverse) {
+    if (elem_major)
+      hipLaunchKernelGGL((line_fft_kernel<true, true>), grid, block, shm, s,
+                         in, out, L, os, Q, qs, es, LB, scale);
+    else
+      hipLaunchKernelGGL((line_fft_kernel<true, false>), grid, block, shm, s,
+                         in, out, L, os, Q, qs, es, LB, scale);
+  } else {
+    if (elem_major)
+      hipLaunchKernelGGL((line_fft_kernel<false, true>), grid, block, shm, s,
+                         in, out, L, os, Q, qs, es, LB, scale);
+    else
+      hipLaunchKernelGGL((line_fft_kernel<false, false>), grid, block, shm, s,
+                         in, out, L, os, Q, qs, es, LB, scale);
+  }
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+int fft3d(const double2* in, double2* out, int64_t B, int n0, int n1, int n2,
+          bool inverse, double scale, const FftLen& Lx, const FftLen& Ly,
+          const FftLen& Lz, hipStream_t s) {
+  int rc = launch_line_fft(in, out, B, n0, n1, n2, 2, inverse, 1.0, Lz, s);
+  if (rc) return rc;
+  rc = launch_line_fft(out, out, B, n0, n1, n2, 1, inverse, 1.0, Ly, s);
+  if (rc) return rc;
+  return launch_line_fft(out, out, B, n0, n1, n2, 0, inverse, scale, Lx, s);
+}
+
+// ------------------------------------------------------- fused, pruned --
+constexpr int kPlaneBlock = 512;
+
+__device__ __forceinline__ int fftfreq_index(int j, int n) {
+  return j < (n + 1) / 2 ? j : j - n;
+}
+
+// Scatter the mask points of one x-plane of one state into LDS, inverse
+// transform along z (only rows holding mask points) and along y, store the
+// plane.
+__global__ __launch_bounds__(kPlaneBlock) void plane_ifft_kernel(
+    const double2* __restrict__ q, double2* __restrict__ T, int64_t state0,
+    BoxGeom geo, FftLen Ly, FftLen Lz) {
+  extern __shared__ double2 sm[];
+  const int ny = geo.ny, nz = geo.nz, P = ny * nz;
+  const int xi = blockIdx.x;
+  const int c = blockIdx.y;
+  const int64_t st = state0 + c;
+  for (int i = threadIdx.x; i < P; i += kPlaneBlock) sm[i] = czero();
+  const int r0 = geo.row_off[xi];
+  const int nr = geo.row_off[xi + 1] - r0;
+  const short* rows = geo.rows + r0;  // y of the rows holding mask points
+  __syncthreads();
+  const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
+  const double2* qs = q + st * geo.ng;
+  for (int g = g0 + threadIdx.x; g < g1; g += kPlaneBlock)
+    sm[geo.g_yz[g]] = qs[g];
+  __syncthreads();
+  lds_fft<true, kPlaneBlock, kPlaneCap, false>(sm, nr, nz, 1, rows, Lz);
+  lds_fft<true, kPlaneBlock, kPlaneCap, true>(sm, nz, 1, nz, nullptr, Ly);
+  double2* dst = T + ((int64_t)c * geo.nxa + xi) * P;
+  for (int i = threadIdx.x; i < P; i += kPlaneBlock) dst[i] = sm[i];
+}
+
+int launch_plane_ifft(const double2* q, double2* T, int64_t state0,
+                      int nstates, const BoxGeom& geo, const FftLen& Ly,
+                      const FftLen& Lz, hipStream_t s) {
+  const int P = geo.ny * geo.nz;
+  size_t shm = (size_t)P * sizeof(double2);
+  dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
+  hipLaunchKernelGGL(plane_ifft_kernel, grid, block, shm, s, q, T, state0, geo,
+                     Ly, Lz);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+constexpr int kXBlock = 256;
+constexpr int kXCap = 2048;
+constexpr int kXEpt = kXCap / kXBlock;
+
+__device__ __forceinline__ void xline_range(int64_t state0, int nstates,
+                                            int nspin_states, int spin,
+                                            int* cb, int* ce) {
+  int64_t b = (int64_t)spin * nspin_states - state0;
+  int64_t e = (int64_t)(spin + 1) * nspin_states - state0;
+  *cb = (int)(b < 0 ? 0 : b);
+  *ce = (int)(e > nstates ? nstates : e);
+}
+
+// One block: fixed y, z in [z0, z0+nl).  Inverse transform along x of every
+// state of the chunk; accumulates occ * |psi|^2 per spin in registers.
+__global__ __launch_bounds__(kXBlock) void xline_density_kernel(
+    const double2* __restrict__ T, double* __restrict__ rho,
+    const double* __restrict__ occ, int64_t state0, int nstates,
+    int nspin_states, double inv_vol, BoxGeom geo, FftLen Lx, int LZ) {
+  extern __shared__ double2 sm[];
+  const int nx = geo.nx, ny = geo.ny, nz = geo.nz, P = ny * nz;
+  int* px = reinterpret_cast<int*>(sm + nx * LZ);
+  const int nzt = (nz + LZ - 1) / LZ;
+  const int y = blockIdx.x / nzt;
+  const int z0 = (blockIdx.x % nzt) * LZ;
+  const int nl = min(LZ, nz - z0);
+  const int spin = blockIdx.y;
+  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
+  int cb, ce;
+  xline_range(state0, nstates, nspin_states, spin, &cb, &ce);
+  if (cb >= ce) return;  // uniform over the block
+  double acc[kXEpt];
+#pragma unroll
+  for (int k = 0; k < kXEpt; ++k) acc[k] = 0.0;
+  const int nxl = nx * LZ;
+  for (int c = cb; c < ce; ++c) {
+    for (int e = threadIdx.x; e < nxl; e += kXBlock) sm[e] = czero();
+    __syncthreads();
+    const double2* src = T + (int64_t)c * geo.nxa * P + (int64_t)y * nz + z0;
+    for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
+      const int l = e % nl, xi = e / nl;
+      sm[px[xi] * LZ + l] = src[(int64_t)xi * P + l];
+    }
+    __syncthreads();
+    lds_fft<true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
+    const double f = occ[state0 + c];
+#pragma unroll
+    for (int k = 0; k < kXEpt; ++k) {
+      const int e = threadIdx.x + k * kXBlock;
+      if (e < nxl && (e % LZ) < nl) {
+        const double2 v = sm[e];
+        acc[k] += f * (v.x * v.x + v.y * v.y);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < kXEpt; ++k) {
+    const int e = threadIdx.x + k * kXBlock;
+    const int x = e / LZ, l = e % LZ;
+    if (e < nxl && l < nl) {
+      const int64_t idx = (((int64_t)spin * nx + x) * ny + y) * nz + z0 + l;
+      rho[idx] += acc[k] * inv_vol;
+    }
+  }
+}
+
+static int xlines_per_block(int nx) {
+  int LZ = kXCap / nx;
+  if (LZ > kXLines) LZ = kXLines;
+  return LZ < 1 ? 1 : LZ;
+}
+
+int launch_xline_density(const double2* T, double* rho, const double* occ,
+                         int64_t state0, int nstates, int nspin_states,
+                         double inv_vol, const BoxGeom& geo, const FftLen& Lx,
+                         hipStream_t s) {
+  if (geo.nx > kXCap) {
+    set_error("x grid size above 2048 is not supported");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  const int LZ = xlines_per_block(geo.nx);
+  const int nzt = (geo.nz + LZ - 1) / LZ;
+  const int nspin = (int)((state0 + nstates - 1) / nspin_states) + 1;
+  size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
+  dim3 grid(geo.ny * nzt, nspin), block(kXBlock);
+  hipLaunchKernelGGL(xline_density_kernel, grid, block, shm, s, T, rho, occ,
+                     state0, nstates, nspin_states, inv_vol, geo, Lx, LZ);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+// T[c] <- FFT_x(v * IFFT_x(T[c])) on the active planes, for every state.
+__global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
+    double2* __restrict__ T, const double* __restrict__ v, int64_t state0,
+    int nstates, int nspin_states, BoxGeom geo, FftLen Lx, int LZ) {
+  extern __shared__ double2 sm[];
+  const int nx = geo.nx, ny = geo.ny, nz = geo.nz, P = ny * nz;
+  int* px = reinterpret_cast<int*>(sm + nx * LZ);
+  const int nzt = (nz + LZ - 1) / LZ;
+  const int y = blockIdx.x / nzt;
+  const int z0 = (blockIdx.x % nzt) * LZ;
+  const int nl = min(LZ, nz - z0);
+  const int spin = blockIdx.y;
+  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
+  int cb, ce;
+  xline_range(state0, nstates, nspin_states, spin, &cb, &ce);
+  if (cb >= ce) return;  // uniform over the block
+  const int nxl = nx * LZ;
+  double vr[kXEpt];
+#pragma unroll
+  for (int k = 0; k < kXEpt; ++k) {
+    const int e = threadIdx.x + k * kXBlock;
+    const int x = e / LZ, l = e % LZ;
+    vr[k] = (e < nxl && l < nl)
+                ? v[(((int64_t)spin * nx + x) * ny + y) * nz + z0 + l]
+                : 0.0;
+  }
+  for (int c = cb; c < ce; ++c) {
+    for (int e = threadIdx.x; e < nxl; e += kXBlock) sm[e] = czero();
+    __syncthreads();
+    double2* buf = T + (int64_t)c * geo.nxa * P + (int64_t)y * nz + z0;
+    for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
+      const int l = e % nl, xi = e / nl;
+      sm[px[xi] * LZ + l] = buf[(int64_t)xi * P + l];
+    }
+    __syncthreads();
+    lds_fft<true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
+#pragma unroll
+    for (int k = 0; k < kXEpt; ++k) {
+      const int e = threadIdx.x + k * kXBlock;
+      if (e < nxl) sm[e] = cscale(sm[e], vr[k]);
+    }
+    __syncthreads();
+    lds_fft<false, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
+    for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
+      const int l = e % nl, xi = e / nl;
+      buf[(int64_t)xi * P + l] = sm[px[xi] * LZ + l];
+    }
+    __syncthreads();
+  }
+}
+
+int launch_xline_vpsi(double2* T, const double* v, int64_t state0,
+                      int nstates, int nspin_states, const BoxGeom& geo,
+                      const FftLen& Lx, hipStream_t s) {
+  if (geo.nx > kXCap) {
+    set_error("x grid size above 2048 is not supported");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  const int LZ = xlines_per_block(geo.nx);
+  const int nzt = (geo.nz + LZ - 1) / LZ;
+  const int nspin = (int)((state0 + nstates - 1) / nspin_states) + 1;
+  size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
+  dim3 grid(geo.ny * nzt, nspin), block(kXBlock);
+  hipLaunchKernelGGL(xline_vpsi_kernel, grid, block, shm, s, T, v, state0,
+                     nstates, nspin_states, geo, Lx, LZ);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+// Forward transform of one plane (y, then z on the mask rows), gather at the
+// mask points and combine with the kinetic term.
+__global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
+    const double2* __restrict__ T, const double2* __restrict__ q,
+    double2* __restrict__ qbar, double* __restrict__ kin_partial,
+    int64_t state0, BoxGeom geo, FftLen Ly, FftLen Lz, GatherParams gp) {
+  extern __shared__ double2 sm[];
+  const int nx = geo.nx, ny = geo.ny, nz = geo.nz, P = ny * nz;
+  const int xi = blockIdx.x;
+  const int c = blockIdx.y;
+  const int64_t st = state0 + c;
+  const double2* src = T + ((int64_t)c * geo.nxa + xi) * P;
+  for (int i = threadIdx.x; i < P; i += kPlaneBlock) sm[i] = src[i];
+  const int r0 = geo.row_off[xi];
+  const int nr = geo.row_off[xi + 1] - r0;
+  const short* rows = geo.rows + r0;
+  __syncthreads();
+  lds_fft<false, kPlaneBlock, kPlaneCap, true>(sm, nz, 1, nz, nullptr, Ly);
+  lds_fft<false, kPlaneBlock, kPlaneCap, false>(sm, nr, nz, 1, rows, Lz);
+  const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
+  const int k = (int)((st / gp.nb) % gp.nk);
+  const double f = gp.occ[st];
+  const double kx = gp.kpts[3 * k], ky = gp.kpts[3 * k + 1],
+               kz = gp.kpts[3 * k + 2];
+  const int mx = fftfreq_index(geo.plane_x[xi], nx);
+  // G + k = mx b1 + my b2 + mz b3 + k
+  const double ax = mx * gp.B[0] + kx, ay = mx * gp.B[1] + ky,
+               az = mx * gp.B[2] + kz;
+  const double2* qs = q + st * geo.ng;
+  double2* qb = qbar + st * geo.ng;
+  double kin = 0.0;
+  for (int g = g0 + threadIdx.x; g < g1; g += kPlaneBlock) {
+    const int yz = geo.g_yz[g];
+    const int iy = yz / nz, iz = yz - iy * nz;
+    const int my = fftfreq_index(iy, ny), mz = fftfreq_index(iz, nz);
+    const double gx = ax + my * gp.B[3] + mz * gp.B[6];
+    const double gy = ay + my * gp.B[4] + mz * gp.B[7];
+    const double gz = az + my * gp.B[5] + mz * gp.B[8];
+    const double gk2 = gx * gx + gy * gy + gz * gz;
+    const double2 cv = qs[g];
+    const double2 hv = sm[yz];
+    qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
+                         f * (gp.two_over_vol * hv.y + gk2 * cv.y));
+    kin += f * gk2 * (cv.x * cv.x + cv.y * cv.y);
+  }
+  __syncthreads();  // the plane is dead: reuse its LDS for the reduction
+  const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
+  if (threadIdx.x == 0) kin_partial[st * geo.nxa + xi] = tot;
+}
+
+int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
+                            double* kin_partial, int64_t state0, int nstates,
+                            const BoxGeom& geo, const FftLen& Ly,
+                            const FftLen& Lz, const GatherParams& gp,
+                            hipStream_t s) {
+  const int P = geo.ny * geo.nz;
+  size_t shm = (size_t)P * sizeof(double2);
+  if (shm < kPlaneBlock * sizeof(double)) shm = kPlaneBlock * sizeof(double);
+  dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
+  hipLaunchKernelGGL(plane_fft_gather_kernel, grid, block, shm, s, T, q, qbar,
+                     kin_partial, state0, geo, Ly, Lz, gp);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+}  // namespace jrx

@@ -1,0 +1,204 @@
+// Workgroup-cooperative mixed-radix FFT on lines held in LDS (FP64).
+//
+// Lines of length n live in LDS; element j of line l is at
+//   buf[base(l) + j*es],  base(l) = (lmap ? lmap[l] : l) * ls.
+// Each stage is a radix-R Stockham autosort pass (decimation in time):
+//   thread-butterfly i in [0, n/R), k = i mod p (p = product of the radices
+//   already applied), u_j = x[i + j n/R] * w^{j k n/(pR)}, y = DFT_R(u),
+//   x'[(i-k) R + k + t p] = y_t.
+// Reads of a stage complete (barrier) before its writes, so one buffer
+// suffices.  All transforms are unnormalised; INV selects e^{+i}.
+#pragma once
+#include "common.h"
+
+namespace jrx {
+
+constexpr int kMaxStages = 12;
+
+struct FftLen {
+  int n;
+  int nstages;
+  int radix[kMaxStages];
+  const double2* tw;  // device, tw[t] = exp(-2 pi i t / n), t in [0, n)
+};
+
+// Factor n into radices 8, 4, 2, 3, 5, 7 (n must be 7-smooth).  Returns
+// false if n has another prime factor.
+inline bool fft_factorize(int n, FftLen* L) {
+  L->n = n;
+  L->nstages = 0;
+  int m = n;
+  auto push = [&](int r) {
+    if (L->nstages >= kMaxStages) return false;
+    L->radix[L->nstages++] = r;
+    return true;
+  };
+  while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
+  while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
+  while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
+  for (int r : {3, 5, 7})
+    while (m % r == 0) { if (!push(r)) return false; m /= r; }
+  return m == 1;
+}
+
+template <bool INV>
+__device__ __forceinline__ double2 twid(const double2* __restrict__ tw,
+                                        int idx) {
+  double2 w = tw[idx];
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// Multiply by -i (forward) or +i (inverse).
+template <bool INV>
+__device__ __forceinline__ double2 mul_mi(double2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+  __device__ __forceinline__ static void run(double2* u, const double2*, int) {
+    double2 a = u[0], b = u[1];
+    u[0] = cadd(a, b);
+    u[1] = csub(a, b);
+  }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+  __device__ __forceinline__ static void run(double2* u, const double2*, int) {
+    double2 a0 = cadd(u[0], u[2]), a1 = csub(u[0], u[2]);
+    double2 b0 = cadd(u[1], u[3]), b1 = mul_mi<INV>(csub(u[1], u[3]));
+    u[0] = cadd(a0, b0);
+    u[2] = csub(a0, b0);
+    u[1] = cadd(a1, b1);
+    u[3] = csub(a1, b1);
+  }
+};
+
+template <bool INV>
+struct Dft<8, INV> {
+  __device__ __forceinline__ static void run(double2* u, const double2*, int) {
+    constexpr double c = 0.70710678118654752440;  // 1/sqrt(2)
+    double2 e[4] = {u[0], u[2], u[4], u[6]};
+    double2 o[4] = {u[1], u[3], u[5], u[7]};
+    Dft<4, INV>::run(e, nullptr, 0);
+    Dft<4, INV>::run(o, nullptr, 0);
+    // o1 *= w8, o2 *= w8^2, o3 *= w8^3 with w8 = exp(-+ i pi/4)
+    double2 t1 = o[1], t3 = o[3];
+    if (!INV) {
+      o[1] = make_double2(c * (t1.x + t1.y), c * (t1.y - t1.x));
+      o[3] = make_double2(c * (t3.y - t3.x), -c * (t3.x + t3.y));
+    } else {
+      o[1] = make_double2(c * (t1.x - t1.y), c * (t1.x + t1.y));
+      o[3] = make_double2(-c * (t3.x + t3.y), c * (t3.x - t3.y));
+    }
+    o[2] = mul_mi<INV>(o[2]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      u[t] = cadd(e[t], o[t]);
+      u[t + 4] = csub(e[t], o[t]);
+    }
+  }
+};
+
+// Odd radices: direct R-point DFT with roots taken from the length-n table
+// (w_R^m = tw[m n/R]), so no separately rounded constants are involved.
+template <int R, bool INV>
+struct DftOdd {
+  __device__ __forceinline__ static void run(double2* u,
+                                             const double2* __restrict__ tw,
+                                             int n) {
+    const int step = n / R;
+    double2 y[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      double2 acc = u[0];
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        acc = cadd(acc, cmul(u[j], twid<INV>(tw, ((j * t) % R) * step)));
+      }
+      y[t] = acc;
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) u[t] = y[t];
+  }
+};
+template <bool INV>
+struct Dft<3, INV> : DftOdd<3, INV> {};
+template <bool INV>
+struct Dft<5, INV> : DftOdd<5, INV> {};
+template <bool INV>
+struct Dft<7, INV> : DftOdd<7, INV> {};
+
+// One radix-R Stockham stage over `nlines` lines.  Requires
+// n * nlines <= CAP and blockDim.x == BLOCK.  LINE_FAST maps consecutive
+// threads to consecutive lines (use when lines are adjacent in LDS, es > 1);
+// otherwise consecutive threads walk along a line (use when es == 1).
+template <int R, bool INV, int BLOCK, int CAP, bool LINE_FAST>
+__device__ __forceinline__ void fft_stage(double2* buf, int n, int nlines,
+                                          int ls, int es,
+                                          const short* lmap, int p,
+                                          const double2* __restrict__ tw) {
+  constexpr int MAXB = (CAP + BLOCK * R - 1) / (BLOCK * R);
+  const int T = n / R;
+  const int total = T * nlines;
+  const int st = n / (p * R);
+  double2 v[MAXB][R];
+#pragma unroll
+  for (int c = 0; c < MAXB; ++c) {
+    const int b = threadIdx.x + c * BLOCK;
+    if (b < total) {
+      const int l = LINE_FAST ? b % nlines : b / T;
+      const int i = LINE_FAST ? b / nlines : b - l * T;
+      const int base = (lmap ? (int)lmap[l] : l) * ls;
+      const int k = i % p;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[c][j] = buf[base + (i + j * T) * es];
+#pragma unroll
+      for (int j = 1; j < R; ++j)
+        v[c][j] = cmul(v[c][j], twid<INV>(tw, j * k * st));
+      Dft<R, INV>::run(v[c], tw, n);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < MAXB; ++c) {
+    const int b = threadIdx.x + c * BLOCK;
+    if (b < total) {
+      const int l = LINE_FAST ? b % nlines : b / T;
+      const int i = LINE_FAST ? b / nlines : b - l * T;
+      const int base = (lmap ? (int)lmap[l] : l) * ls;
+      const int k = i % p;
+      const int o = (i - k) * R + k;
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[base + (o + t * p) * es] = v[c][t];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform of `nlines` lines.  Caller guarantees L.n * nlines <= CAP
+// and that the buffer is visible to all threads (barrier before the call).
+template <bool INV, int BLOCK, int CAP, bool LINE_FAST>
+__device__ void lds_fft(double2* buf, int nlines, int ls, int es,
+                        const short* lmap, const FftLen& L) {
+  int p = 1;
+  for (int s = 0; s < L.nstages; ++s) {
+    const int R = L.radix[s];
+    switch (R) {
+      case 8: fft_stage<8, INV, BLOCK, CAP, LINE_FAST>(buf, L.n, nlines, ls, es, lmap, p, L.tw); break;
+      case 4: fft_stage<4, INV, BLOCK, CAP, LINE_FAST>(buf, L.n, nlines, ls, es, lmap, p, L.tw); break;
+      case 2: fft_stage<2, INV, BLOCK, CAP, LINE_FAST>(buf, L.n, nlines, ls, es, lmap, p, L.tw); break;
+      case 3: fft_stage<3, INV, BLOCK, CAP, LINE_FAST>(buf, L.n, nlines, ls, es, lmap, p, L.tw); break;
+      case 5: fft_stage<5, INV, BLOCK, CAP, LINE_FAST>(buf, L.n, nlines, ls, es, lmap, p, L.tw); break;
+      default: fft_stage<7, INV, BLOCK, CAP, LINE_FAST>(buf, L.n, nlines, ls, es, lmap, p, L.tw); break;
+    }
+    p *= R;
+  }
+}
+
+}  // namespace jrx

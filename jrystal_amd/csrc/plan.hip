@@ -1,0 +1,521 @@
+// Plan, workspace layout and orchestration of one energy(+gradient)
+// evaluation; C ABI entry points.
+#include <math.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "energy_kernels.h"
+#include "fft_kernels.h"
+#include "qr_kernels.h"
+
+namespace jrx {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Twiddle table exp(-2 pi i t / n), t in [0, n), in extended precision.
+static std::vector<double2> host_twiddles(int n) {
+  std::vector<double2> tw(n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < n; ++t) {
+    const long double a = two_pi * (long double)t / (long double)n;
+    tw[t] = make_double2((double)cosl(a), (double)(-sinl(a)));
+  }
+  return tw;
+}
+
+// Per-device cache of twiddle tables for the plan-less FFT entry points.
+static std::mutex g_tw_mutex;
+static std::map<std::pair<int, int>, double2*> g_tw_cache;
+
+static int cached_fftlen(int n, FftLen* L) {
+  if (n < 1 || !fft_factorize(n, L)) {
+    set_error("FFT size " + std::to_string(n) +
+              " is not a product of 2, 3, 5 and 7");
+    return JRX_ERR_INVALID;
+  }
+  if (n > 2048) {
+    set_error("FFT size above 2048 is not supported");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  int dev = 0;
+  JRX_CHECK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tw_mutex);
+  auto it = g_tw_cache.find({dev, n});
+  if (it == g_tw_cache.end()) {
+    std::vector<double2> h = host_twiddles(n);
+    double2* d = nullptr;
+    JRX_CHECK_HIP(hipMalloc(&d, n * sizeof(double2)));
+    JRX_CHECK_HIP(hipMemcpy(d, h.data(), n * sizeof(double2),
+                            hipMemcpyHostToDevice));
+    it = g_tw_cache.emplace(std::make_pair(dev, n), d).first;
+  }
+  L->tw = it->second;
+  return JRX_OK;
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace jrx
+
+using namespace jrx;
+
+struct jrx_plan {
+  int device = 0;
+  int ns = 0, nk = 0, nb = 0, nx = 0, ny = 0, nz = 0, ng = 0, natom = 0;
+  int64_t P = 0, S = 0, N = 0;
+  double vol = 0, cx = 0;
+  double B[9];
+  std::vector<void*> allocs;
+  BoxGeom geo;
+  FftLen Lx, Ly, Lz;
+  CellParams cp;
+  short* d_g_m = nullptr;
+  double* d_kpts = nullptr;
+  double* d_occ = nullptr;
+  int chunk = 0;
+  GramSplit gsw, gsq;
+  int kin_parts_max = 0, rec_parts = 0;
+  // timing
+  int timing = 0;
+  std::vector<hipEvent_t> events;
+  std::vector<std::pair<int, int>> records;  // (class, event pair index)
+  // workspace offsets (bytes)
+  size_t o_q, o_qbar, o_T, o_rho, o_v, o_box, o_S, o_R1, o_R2, o_Ri, o_A,
+      o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, total;
+};
+
+namespace {
+
+template <typename T>
+int upload(jrx_plan* p, const T* h, size_t n, T** out) {
+  T* d = nullptr;
+  JRX_CHECK_HIP(hipMalloc(&d, n * sizeof(T) + 16));
+  p->allocs.push_back(d);
+  if (n) JRX_CHECK_HIP(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  *out = d;
+  return JRX_OK;
+}
+
+int make_len(jrx_plan* p, int n, FftLen* L) {
+  if (!fft_factorize(n, L)) {
+    set_error("grid size " + std::to_string(n) +
+              " is not a product of 2, 3, 5 and 7");
+    return JRX_ERR_INVALID;
+  }
+  std::vector<double2> tw = host_twiddles(n);
+  double2* d = nullptr;
+  int rc = upload(p, tw.data(), tw.size(), &d);
+  L->tw = d;
+  return rc;
+}
+
+void det_inv3(const double* a, double* det, double* inv) {
+  const double c00 = a[4] * a[8] - a[5] * a[7];
+  const double c01 = a[5] * a[6] - a[3] * a[8];
+  const double c02 = a[3] * a[7] - a[4] * a[6];
+  *det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  const double id = 1.0 / *det;
+  inv[0] = c00 * id;
+  inv[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  inv[3] = c01 * id;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  inv[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  inv[6] = c02 * id;
+  inv[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+int plan_build(jrx_plan* p, const jrx_problem* d) {
+  JRX_REQUIRE(d->ns == 1 || d->ns == 2, "ns must be 1 or 2");
+  JRX_REQUIRE(d->nk >= 1 && d->nb >= 1, "nk and nb must be positive");
+  JRX_REQUIRE((int64_t)d->ns * d->nk <= 65535, "ns*nk above 65535");
+  JRX_REQUIRE(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "grid sizes must be positive");
+  JRX_REQUIRE(d->mask && d->kpts && d->cell && d->occupation,
+              "mask, kpts, cell and occupation are required");
+  JRX_REQUIRE(d->natom >= 0 && (d->natom == 0 || (d->positions && d->charges)),
+              "positions and charges are required");
+  JRX_REQUIRE(d->xc == JRX_XC_LDA_X, "only xc = lda_x is supported");
+  if (d->nb > kMaxBands) {
+    set_error("nb above 64 is not supported by this build");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  if ((int64_t)d->ny * d->nz > kPlaneCap) {
+    set_error("ny*nz above 6400 is not supported by this build");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  JRX_CHECK_HIP(hipGetDevice(&p->device));
+  p->ns = d->ns; p->nk = d->nk; p->nb = d->nb;
+  p->nx = d->nx; p->ny = d->ny; p->nz = d->nz; p->natom = d->natom;
+  p->P = (int64_t)d->ns * d->nk;
+  p->S = p->P * d->nb;
+  p->N = (int64_t)d->nx * d->ny * d->nz;
+  // Cell: vol = |det A|, B = 2 pi A^{-T}
+  double det, inv[9];
+  det_inv3(d->cell, &det, inv);
+  p->vol = fabs(det);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) p->B[3 * i + j] = 2.0 * M_PI * inv[3 * j + i];
+  p->cx = 0.75 * cbrt(3.0 / M_PI);
+  // Geometry of the mask.
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  std::vector<int> plane_g0, plane_x, g_yz, row_off;
+  std::vector<short> rows, g_m;
+  int ng = 0, max_rows = 0;
+  auto fq = [](int j, int n) { return j < (n + 1) / 2 ? j : j - n; };
+  for (int x = 0; x < nx; ++x) {
+    int first = -1;
+    std::vector<short> prow;
+    for (int y = 0; y < ny; ++y) {
+      bool any = false;
+      for (int z = 0; z < nz; ++z) {
+        if (d->mask[((size_t)x * ny + y) * nz + z]) {
+          if (first < 0) first = ng;
+          g_yz.push_back(y * nz + z);
+          g_m.push_back((short)fq(x, nx));
+          g_m.push_back((short)fq(y, ny));
+          g_m.push_back((short)fq(z, nz));
+          ++ng;
+          any = true;
+        }
+      }
+      if (any) prow.push_back((short)y);
+    }
+    if (first >= 0) {
+      plane_g0.push_back(first);
+      plane_x.push_back(x);
+      row_off.push_back((int)rows.size());
+      rows.insert(rows.end(), prow.begin(), prow.end());
+      if ((int)prow.size() > max_rows) max_rows = (int)prow.size();
+    }
+  }
+  JRX_REQUIRE(ng > 0, "mask selects no G vectors");
+  JRX_REQUIRE(nx <= 2047 && ny <= 2047 && nz <= 2047, "grid too large");
+  plane_g0.push_back(ng);
+  row_off.push_back((int)rows.size());
+  p->ng = ng;
+  p->geo.nx = nx; p->geo.ny = ny; p->geo.nz = nz; p->geo.ng = ng;
+  p->geo.nxa = (int)plane_x.size();
+  p->geo.max_rows = max_rows;
+  int rc;
+  int* tmp = nullptr;
+  if ((rc = upload(p, plane_g0.data(), plane_g0.size(), &tmp))) return rc;
+  p->geo.plane_g0 = tmp;
+  if ((rc = upload(p, plane_x.data(), plane_x.size(), &tmp))) return rc;
+  p->geo.plane_x = tmp;
+  if ((rc = upload(p, g_yz.data(), g_yz.size(), &tmp))) return rc;
+  p->geo.g_yz = tmp;
+  if ((rc = upload(p, row_off.data(), row_off.size(), &tmp))) return rc;
+  p->geo.row_off = tmp;
+  short* srows = nullptr;
+  if ((rc = upload(p, rows.data(), rows.size(), &srows))) return rc;
+  p->geo.rows = srows;
+  if ((rc = upload(p, g_m.data(), g_m.size(), &p->d_g_m))) return rc;
+  if ((rc = make_len(p, nx, &p->Lx))) return rc;
+  if ((rc = make_len(p, ny, &p->Ly))) return rc;
+  if ((rc = make_len(p, nz, &p->Lz))) return rc;
+  if ((rc = upload(p, d->kpts, (size_t)d->nk * 3, &p->d_kpts))) return rc;
+  if ((rc = upload(p, d->occupation, (size_t)p->S, &p->d_occ))) return rc;
+  double *dpos = nullptr, *dchg = nullptr;
+  if ((rc = upload(p, d->positions, (size_t)d->natom * 3, &dpos))) return rc;
+  if ((rc = upload(p, d->charges, (size_t)d->natom, &dchg))) return rc;
+  p->cp.nx = nx; p->cp.ny = ny; p->cp.nz = nz;
+  for (int i = 0; i < 9; ++i) p->cp.B[i] = p->B[i];
+  p->cp.vol = p->vol;
+  p->cp.natom = d->natom;
+  p->cp.pos = dpos;
+  p->cp.charge = dchg;
+  // LDS requirements against the device limit.
+  int max_lds = 0;
+  JRX_CHECK_HIP(hipDeviceGetAttribute(
+      &max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+  const int nb = d->nb;
+  const size_t lds_plane = (size_t)ny * nz * sizeof(double2);
+  const size_t lds_qr = ((size_t)kQrRows * (nb + 1) + (size_t)nb * (kQrRows + 1)) *
+                        sizeof(double2);
+  const size_t lds_gram = ((size_t)kQrRows * (nb + 1) + (size_t)nb * nb) *
+                          sizeof(double2);
+  const size_t lds_chol = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
+  size_t need = lds_plane;
+  for (size_t v : {lds_qr, lds_gram, lds_chol}) need = v > need ? v : need;
+  if (need > (size_t)max_lds) {
+    set_error("problem needs " + std::to_string(need) +
+              " bytes of LDS per workgroup; device allows " +
+              std::to_string(max_lds));
+    return JRX_ERR_UNSUPPORTED;
+  }
+  // Chunking of the FFT sweeps.
+  const size_t per_state = (size_t)p->geo.nxa * ny * nz * sizeof(double2);
+  int64_t chunk = d->chunk_states;
+  if (chunk <= 0) {
+    const size_t budget = (size_t)1 << 30;
+    chunk = (int64_t)(budget / per_state);
+    if (chunk < 1) chunk = 1;
+  }
+  if (chunk > p->S) chunk = p->S;
+  if (chunk > 65535) chunk = 65535;
+  p->chunk = (int)chunk;
+  p->gsw = gram_split(p->P, ng);
+  p->gsq = p->gsw;
+  const int64_t kin_gather = p->S * p->geo.nxa;
+  const int64_t kin_plain = p->S * ((ng + kRedBlock - 1) / kRedBlock);
+  p->kin_parts_max = (int)(kin_gather > kin_plain ? kin_gather : kin_plain);
+  p->rec_parts = (int)((p->N + kRedBlock - 1) / kRedBlock);
+  // Workspace layout.
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes);
+    return o;
+  };
+  const size_t mat = (size_t)p->P * nb * nb * sizeof(double2);
+  p->o_q = take((size_t)p->S * ng * sizeof(double2));
+  p->o_qbar = take((size_t)p->S * ng * sizeof(double2));
+  p->o_T = take((size_t)p->chunk * per_state);
+  p->o_rho = take((size_t)p->ns * p->N * sizeof(double));
+  p->o_v = take((size_t)p->ns * p->N * sizeof(double));
+  p->o_box = take((size_t)p->N * sizeof(double2));
+  p->o_S = take(mat);
+  p->o_R1 = take(mat);
+  p->o_R2 = take(mat);
+  p->o_Ri = take(mat);
+  p->o_A = take(mat);
+  p->o_Bm = take(mat);
+  p->o_part = take((size_t)p->P * p->gsw.nchunk * nb * nb * sizeof(double2));
+  p->o_kin = take((size_t)p->kin_parts_max * sizeof(double));
+  p->o_eh = take((size_t)p->rec_parts * sizeof(double));
+  p->o_ee = take((size_t)p->rec_parts * sizeof(double));
+  p->o_exc = take((size_t)p->rec_parts * sizeof(double));
+  p->total = off;
+  return JRX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jrx_last_error(void) { return g_last_error.c_str(); }
+const char* jrx_version(void) { return "jrx-hip 0.1 (gfx950, fp64)"; }
+
+static int generic_fft(const double* x, double* y, int64_t batch, int nx,
+                       int ny, int nz, double scale, void* stream, bool inv) {
+  JRX_REQUIRE(batch >= 0 && nx > 0 && ny > 0 && nz > 0, "invalid FFT shape");
+  JRX_REQUIRE(x && y, "null FFT buffer");
+  if (batch == 0) return JRX_OK;
+  FftLen Lx, Ly, Lz;
+  int rc;
+  if ((rc = cached_fftlen(nx, &Lx)) || (rc = cached_fftlen(ny, &Ly)) ||
+      (rc = cached_fftlen(nz, &Lz)))
+    return rc;
+  return fft3d(reinterpret_cast<const double2*>(x),
+               reinterpret_cast<double2*>(y), batch, nx, ny, nz, inv, scale,
+               Lx, Ly, Lz, (hipStream_t)stream);
+}
+
+int jrx_fftn3d(const double* x, double* y, int64_t batch, int nx, int ny,
+               int nz, double scale, void* stream) {
+  return generic_fft(x, y, batch, nx, ny, nz, scale, stream, false);
+}
+
+int jrx_ifftn3d(const double* x, double* y, int64_t batch, int nx, int ny,
+                int nz, double scale, void* stream) {
+  return generic_fft(x, y, batch, nx, ny, nz, scale, stream, true);
+}
+
+int jrx_plan_create(jrx_plan** plan, const jrx_problem* problem) {
+  JRX_REQUIRE(plan && problem, "null argument");
+  jrx_plan* p = new jrx_plan();
+  int rc = plan_build(p, problem);
+  if (rc) {
+    for (void* a : p->allocs) (void)hipFree(a);
+    delete p;
+    *plan = nullptr;
+    return rc;
+  }
+  *plan = p;
+  return JRX_OK;
+}
+
+int jrx_plan_destroy(jrx_plan* plan) {
+  if (!plan) return JRX_OK;
+  for (hipEvent_t e : plan->events) (void)hipEventDestroy(e);
+  for (void* a : plan->allocs) (void)hipFree(a);
+  delete plan;
+  return JRX_OK;
+}
+
+int jrx_plan_workspace_bytes(const jrx_plan* plan, size_t* bytes) {
+  JRX_REQUIRE(plan && bytes, "null argument");
+  *bytes = plan->total;
+  return JRX_OK;
+}
+
+int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info) {
+  JRX_REQUIRE(plan && info, "null argument");
+  info->ng = plan->ng;
+  info->active_planes = plan->geo.nxa;
+  info->chunk_states = plan->chunk;
+  info->workspace_bytes = plan->total;
+  return JRX_OK;
+}
+
+int jrx_plan_set_timing(jrx_plan* plan, int enable) {
+  JRX_REQUIRE(plan, "null plan");
+  plan->timing = enable ? 1 : 0;
+  return JRX_OK;
+}
+
+int jrx_plan_read_timing(jrx_plan* plan, double* ms, int* launches) {
+  JRX_REQUIRE(plan && ms && launches, "null argument");
+  for (int c = 0; c < JRX_K_COUNT; ++c) {
+    ms[c] = 0.0;
+    launches[c] = 0;
+  }
+  for (const auto& r : plan->records) {
+    hipEvent_t a = plan->events[2 * r.second], b = plan->events[2 * r.second + 1];
+    JRX_CHECK_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    JRX_CHECK_HIP(hipEventElapsedTime(&t, a, b));
+    ms[r.first] += t;
+    launches[r.first] += 1;
+  }
+  plan->records.clear();
+  return JRX_OK;
+}
+
+static int timing_begin(jrx_plan* p, hipStream_t s, int* slot) {
+  *slot = -1;
+  if (!p->timing) return JRX_OK;
+  const int i = (int)p->records.size();
+  while ((int)p->events.size() < 2 * (i + 1)) {
+    hipEvent_t e;
+    JRX_CHECK_HIP(hipEventCreate(&e));
+    p->events.push_back(e);
+  }
+  JRX_CHECK_HIP(hipEventRecord(p->events[2 * i], s));
+  *slot = i;
+  return JRX_OK;
+}
+
+static int timing_end(jrx_plan* p, hipStream_t s, int slot, int cls) {
+  if (slot < 0) return JRX_OK;
+  JRX_CHECK_HIP(hipEventRecord(p->events[2 * slot + 1], s));
+  p->records.push_back({cls, slot});
+  return JRX_OK;
+}
+
+int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
+                    double* energies, double* dw_re, double* dw_im,
+                    double* rho_out, double* q_out, void* workspace,
+                    size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && w_re && w_im && energies, "null argument");
+  JRX_REQUIRE((dw_re == nullptr) == (dw_im == nullptr),
+              "dw_re and dw_im must both be given or both be NULL");
+  JRX_REQUIRE(workspace && workspace_bytes >= p->total,
+              "workspace too small (see jrx_plan_workspace_bytes)");
+  JRX_REQUIRE(((uintptr_t)workspace & 255) == 0,
+              "workspace must be 256-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = static_cast<char*>(workspace);
+  auto cptr = [&](size_t o) { return reinterpret_cast<double2*>(ws + o); };
+  auto dptr = [&](size_t o) { return reinterpret_cast<double*>(ws + o); };
+  double2* q = cptr(p->o_q);
+  double2* qbar = cptr(p->o_qbar);
+  double2* T = cptr(p->o_T);
+  double* rho = dptr(p->o_rho);
+  double* v = dptr(p->o_v);
+  double2* box = cptr(p->o_box);
+  double2* Sm = cptr(p->o_S);
+  double2* R1 = cptr(p->o_R1);
+  double2* R2 = cptr(p->o_R2);
+  double2* Ri = cptr(p->o_Ri);
+  double2* Am = cptr(p->o_A);
+  double2* Bm = cptr(p->o_Bm);
+  double2* part = cptr(p->o_part);
+  double* kin = dptr(p->o_kin);
+  double* eh = dptr(p->o_eh);
+  double* ee = dptr(p->o_ee);
+  double* exc = dptr(p->o_exc);
+  const int nb = p->nb, ng = p->ng;
+  const int64_t P = p->P, S = p->S;
+  int rc;
+  int slot;
+#define JRX_T(cls, x)                                   \
+  do {                                                  \
+    if ((rc = timing_begin(p, s, &slot))) return rc;    \
+    if ((rc = (x))) return rc;                          \
+    if ((rc = timing_end(p, s, slot, cls))) return rc;  \
+  } while (0)
+  // --- QR (CholeskyQR2) ----------------------------------------------------
+  JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, p->gsw, s));
+  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s));
+  JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s));
+  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));
+  JRX_T(JRX_K_QR_APPLY, launch_apply_fwd(w_re, w_im, Ri, q, P, ng, nb, s));
+  // --- density -------------------------------------------------------------
+  JRX_T(JRX_K_OTHER, launch_fill(rho, 0.0, (int64_t)p->ns * p->N, s));
+  const int nspin_states = p->nk * nb;
+  for (int64_t s0 = 0; s0 < S; s0 += p->chunk) {
+    const int nst = (int)((S - s0) < p->chunk ? (S - s0) : p->chunk);
+    JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, T, s0, nst, p->geo, p->Ly, p->Lz, s));
+    JRX_T(JRX_K_XLINE_DENSITY, launch_xline_density(T, rho, p->d_occ, s0, nst, nspin_states,
+                                 1.0 / p->vol, p->geo, p->Lx, s));
+  }
+  // --- reciprocal energies and potential ----------------------------------
+  JRX_T(JRX_K_POTENTIAL, launch_rho_to_complex(rho, box, p->ns, p->N, s));
+  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, p->nx, p->ny, p->nz, false, 1.0, p->Lx, p->Ly,
+                p->Lz, s));
+  int nrec = 0, nexc = 0, nkin = 0;
+  JRX_T(JRX_K_POTENTIAL, launch_potential_rec(box, box, eh, ee, p->cp, &nrec, s));
+  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, p->nx, p->ny, p->nz, true, 1.0 / (double)p->N,
+                p->Lx, p->Ly, p->Lz, s));
+  JRX_T(JRX_K_POTENTIAL, launch_potential_real(box, rho, v, exc, p->ns, p->N, p->vol, p->cx,
+                                &nexc, s));
+  if (dw_re) {
+    // --- adjoint sweep: qbar = f (2/vol FFT(v psi) + |G+k|^2 q) -------------
+    GatherParams gp;
+    gp.occ = p->d_occ;
+    gp.kpts = p->d_kpts;
+    for (int i = 0; i < 9; ++i) gp.B[i] = p->B[i];
+    gp.nk = p->nk;
+    gp.nb = nb;
+    gp.two_over_vol = 2.0 / p->vol;
+    for (int64_t s0 = 0; s0 < S; s0 += p->chunk) {
+      const int nst = (int)((S - s0) < p->chunk ? (S - s0) : p->chunk);
+      JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, T, s0, nst, p->geo, p->Ly, p->Lz, s));
+      JRX_T(JRX_K_XLINE_VPSI, launch_xline_vpsi(T, v, s0, nst, nspin_states, p->geo, p->Lx, s));
+      JRX_T(JRX_K_PLANE_FFT_GATHER, launch_plane_fft_gather(T, q, qbar, kin, s0, nst, p->geo, p->Ly,
+                                      p->Lz, gp, s));
+    }
+    nkin = (int)(S * p->geo.nxa);
+    // --- QR adjoint ----------------------------------------------------------
+    JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
+    JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsq.nchunk, nb, s));
+    JRX_T(JRX_K_QR_SMALL, launch_qr_bwd_small(Sm, Ri, Am, Bm, P, nb, s));
+    JRX_T(JRX_K_QR_BWD_APPLY, launch_apply_bwd(qbar, q, Am, Bm, dw_re, dw_im, P, ng, nb, s));
+  } else {
+    JRX_T(JRX_K_OTHER, launch_kinetic(q, p->d_occ, p->d_kpts, p->d_g_m, p->cp, S, ng,
+                           p->nk, nb, kin, &nkin, s));
+  }
+  JRX_T(JRX_K_OTHER, launch_finalize_energies(kin, nkin, eh, ee, nrec, exc, nexc,
+                                   energies, s));
+  if (rho_out)
+    JRX_CHECK_HIP(hipMemcpyAsync(rho_out, rho,
+                                 (size_t)p->ns * p->N * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+  if (q_out)
+    JRX_CHECK_HIP(hipMemcpyAsync(q_out, q, (size_t)S * ng * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, s));
+#undef JRX_T
+  return JRX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+"""Fused evaluation of the direct-minimisation objective on the GPU.
+
+``EnergyPlan`` owns a ``jrx_plan`` (libjrx_hip.so) for one problem and
+evaluates, in one stream-ordered sequence of HIP kernels,
+
+  E(w) = E_kin + E_H + E_ext + E_xc     and     dE/dw_re, dE/dw_im
+
+for the parameters w = {'w_re', 'w_im'} of layout (spin, kpt, g, band).
+This is the hot path of jrystal's all-electron driver:
+``total_energy`` + ``jax.value_and_grad`` in
+jrystal/calc/calc_ground_state_energy_all_electrons.py:119-137,175-181, for
+fixed (uniform / gamma) occupations, whose gradient is zero (occupation.py:
+271-274 applies stop_gradient).
+
+``total_energy(plan, w_re, w_im)`` is the autograd-aware form: its backward
+returns the gradient that the same fused call computed.
+"""
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+
+ENERGY_KEYS = ("kinetic", "hartree", "external", "xc", "total")
+
+
+def _require_cuda():
+  if not torch.cuda.is_available():
+    raise RuntimeError("the jrystal_amd GPU path needs a ROCm GPU "
+                       "(no CPU fallback)")
+
+
+class EnergyPlan:
+  """Plan for one (crystal, mask, k-points, occupation) problem.
+
+  Args:
+    positions: (natom, 3) bohr.  charges: (natom,).  cell_vectors: (3, 3)
+    bohr, rows.  mask: (nx, ny, nz) bool.  kpts: (nk, 3).  occupation:
+    (ns, nk, nb).  chunk_states: states per FFT chunk (0 = automatic).
+  """
+
+  def __init__(self, positions, charges, cell_vectors, mask, kpts,
+               occupation, xc: str = "lda_x", chunk_states: int = 0,
+               device=None):
+    _require_cuda()
+    if xc != "lda_x":
+      raise NotImplementedError(f"xc {xc!r}: only lda_x is on this path")
+    self.device = torch.device(device if device is not None else "cuda")
+    self._lib = _lib.lib()
+    mask = np.ascontiguousarray(np.asarray(mask, dtype=bool))
+    if mask.ndim != 3:
+      raise ValueError("mask must be a 3D boolean array")
+    occ = np.ascontiguousarray(np.asarray(occupation, np.float64))
+    if occ.ndim != 3:
+      raise ValueError("occupation must have shape (spin, kpt, band)")
+    kpts = np.ascontiguousarray(np.asarray(kpts, np.float64).reshape(-1, 3))
+    if kpts.shape[0] != occ.shape[1]:
+      raise ValueError(
+        "Occupation should have a leading dimension that is the same as "
+        f"coeff. Got occupation shape: {occ.shape}, kpts: {kpts.shape}")
+    pos = np.ascontiguousarray(np.asarray(positions, np.float64).reshape(-1, 3))
+    chg = np.ascontiguousarray(np.asarray(charges, np.float64).reshape(-1))
+    cell = np.ascontiguousarray(np.asarray(cell_vectors, np.float64))
+    self._keep = (mask.astype(np.uint8), occ, kpts, pos, chg, cell)
+    m8 = self._keep[0]
+    d = _lib.JrxProblem()
+    d.ns, d.nk, d.nb = occ.shape
+    d.nx, d.ny, d.nz = mask.shape
+    d.natom = pos.shape[0]
+    d.mask = m8.ctypes.data
+    d.kpts = kpts.ctypes.data
+    d.cell = cell.ctypes.data
+    d.positions = pos.ctypes.data
+    d.charges = chg.ctypes.data
+    d.occupation = occ.ctypes.data
+    d.xc = 0
+    d.chunk_states = int(chunk_states)
+    h = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_create(ctypes.byref(h), ctypes.byref(d)))
+    self._plan = h
+    info = _lib.JrxPlanInfo()
+    _lib.check(self._lib.jrx_plan_get_info(h, ctypes.byref(info)))
+    self.num_g = info.ng
+    self.active_planes = info.active_planes
+    self.chunk_states = info.chunk_states
+    self.ns, self.nk, self.nb = occ.shape
+    self.grid_sizes = mask.shape
+    self.param_shape = (self.ns, self.nk, self.num_g, self.nb)
+    self.workspace_bytes = info.workspace_bytes
+    # torch's caching allocator returns >= 512-byte aligned blocks.
+    self._ws = torch.empty(self.workspace_bytes + 256, dtype=torch.uint8,
+                           device=self.device)
+    self._ws_ptr = (self._ws.data_ptr() + 255) // 256 * 256
+
+  @classmethod
+  def from_workload(cls, wl, chunk_states=0, device=None):
+    return cls(wl.crystal.positions, wl.crystal.charges,
+               wl.crystal.cell_vectors, wl.mask, wl.kpts, wl.occupation,
+               chunk_states=chunk_states, device=device)
+
+  def close(self):
+    if getattr(self, "_plan", None) is not None and self._plan.value:
+      self._lib.jrx_plan_destroy(self._plan)
+      self._plan = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:
+      pass
+
+  def _as_param(self, x, name):
+    if isinstance(x, np.ndarray):
+      x = torch.from_numpy(x)
+    if tuple(x.shape) != self.param_shape:
+      raise ValueError(f"{name} has shape {tuple(x.shape)}, expected "
+                       f"{self.param_shape}")
+    return x.to(device=self.device, dtype=torch.float64).contiguous()
+
+  def evaluate(self, w_re, w_im, grad: bool = True, density: bool = False,
+               coefficients: bool = False, out: Optional[dict] = None):
+    """Run one evaluation on the current stream.  Returns a dict with
+    'energies' (device tensor [5]: kinetic, hartree, external, xc, total),
+    and when requested 'dw_re', 'dw_im', 'density' (ns,nx,ny,nz) and
+    'coefficients' (ns,nk,nb,ng complex128)."""
+    w_re = self._as_param(w_re, "w_re")
+    w_im = self._as_param(w_im, "w_im")
+    out = {} if out is None else out
+    dev = self.device
+    e = out.get("energies")
+    if e is None:
+      e = out["energies"] = torch.empty(5, dtype=torch.float64, device=dev)
+    dwr = dwi = rho = q = None
+    if grad:
+      dwr = out.get("dw_re")
+      if dwr is None:
+        dwr = out["dw_re"] = torch.empty_like(w_re)
+      dwi = out.get("dw_im")
+      if dwi is None:
+        dwi = out["dw_im"] = torch.empty_like(w_im)
+    if density:
+      rho = out["density"] = torch.empty((self.ns,) + tuple(self.grid_sizes),
+                                         dtype=torch.float64, device=dev)
+    if coefficients:
+      q = out["coefficients"] = torch.empty(
+        (self.ns, self.nk, self.nb, self.num_g), dtype=torch.complex128,
+        device=dev)
+    with torch.cuda.device(dev):
+      rc = self._lib.jrx_energy_grad(
+        self._plan, _lib.ptr(w_re), _lib.ptr(w_im), _lib.ptr(e),
+        _lib.ptr(dwr), _lib.ptr(dwi), _lib.ptr(rho), _lib.ptr(q),
+        self._ws_ptr, self.workspace_bytes, _lib.stream_ptr(dev))
+    _lib.check(rc)
+    return out
+
+  def set_timing(self, enable: bool):
+    _lib.check(self._lib.jrx_plan_set_timing(self._plan, 1 if enable else 0))
+
+  def read_timing(self):
+    """Per-kernel-class (ms, launches) since the last read (synchronises)."""
+    n = len(_lib.KERNEL_CLASSES)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_int * n)()
+    _lib.check(self._lib.jrx_plan_read_timing(self._plan, ms, cnt))
+    return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KERNEL_CLASSES)}
+
+  def value_and_grad(self, params):
+    """(energies dict of floats, {'w_re': dE/dw_re, 'w_im': dE/dw_im})."""
+    r = self.evaluate(params["w_re"], params["w_im"], grad=True)
+    vals = r["energies"].tolist()
+    return dict(zip(ENERGY_KEYS, vals)), {"w_re": r["dw_re"],
+                                          "w_im": r["dw_im"]}
+
+
+class _TotalEnergy(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, plan, w_re, w_im):
+    r = plan.evaluate(w_re, w_im, grad=True)
+    ctx.save_for_backward(r["dw_re"], r["dw_im"])
+    return r["energies"][4].clone()
+
+  @staticmethod
+  def backward(ctx, g):
+    dwr, dwi = ctx.saved_tensors
+    return None, g * dwr, g * dwi
+
+
+def total_energy(plan: EnergyPlan, w_re, w_im):
+  """Differentiable total electronic energy (no Ewald term)."""
+  return _TotalEnergy.apply(plan, w_re, w_im)

@@ -1,0 +1,163 @@
+"""Energy and closed-form gradient of the direct-minimisation objective.
+
+Restates the closed ``total_energy`` of
+jrystal/calc/calc_ground_state_energy_all_electrons.py:119-137
+(kinetic + hartree + external + xc for fixed occupations) and the gradient
+that ``jax.value_and_grad`` returns for it (:178), written as the explicit
+adjoint chain:
+
+  v(r)      = dE/drho(r): Hartree + external + LDA-x potentials, * vol/N
+  Cbar      = f [ (2/sqrt(vol)) fftn(v psi) + |G+k|^2 C ]   (torch convention:
+              dE/dRe C + i dE/dIm C)
+  Qbar      = Cbar gathered at the mask
+  Wbar      = (Qbar + Q copyltu(M)) R^{-H},  copyltu(M) = X + X^H - Re diag X,
+              X = triu(-Q^H Qbar)     (QR backward; Walter & Lehmann 2018,
+              Liao et al. PRX 9 031041 (2019))
+  dE/dw_re  = Re Wbar,  dE/dw_im = Im Wbar.
+
+Works k-chunk by k-chunk so that the cfg2 shape (64 k x 32 bands x 64^3)
+fits in host memory.  Pinned in tests against torch CPU autograd (an
+independent AD system) and central finite differences.
+"""
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import scipy.fft as sfft
+
+from . import energy as _en
+from . import pw as _pw
+
+
+@dataclass
+class Problem:
+  """Everything the objective needs besides the parameters."""
+  mask: np.ndarray          # (x,y,z) bool
+  g_vector_grid: np.ndarray  # (x,y,z,3)
+  kpts: np.ndarray          # (nk,3)
+  vol: float
+  occupation: np.ndarray    # (s,k,b)
+  positions: np.ndarray     # (atom,3)
+  charges: np.ndarray       # (atom,)
+  xc: str = "lda_x"
+
+  @property
+  def grid_sizes(self):
+    return self.mask.shape
+
+  def mask_g(self):
+    """G vectors at the mask points in C order, (ng,3)."""
+    return self.g_vector_grid[self.mask]
+
+
+def potential_real(rho, prob: Problem):
+  """v_s(r) = dE/drho_s(r) for E = E_H + E_ext + E_xc (vol/N included).
+
+  Returns (v (s,x,y,z), rho_rec (s,x,y,z), energies dict)."""
+  g = prob.g_vector_grid
+  vol = prob.vol
+  n = np.prod(rho.shape[-3:])
+  rho_rec = sfft.fftn(rho, axes=(-3, -2, -1), workers=_pw._WORKERS)
+  e_har = _en.hartree(rho_rec, g, vol)
+  e_ext = _en.external(rho_rec, prob.positions, prob.charges, g, vol)
+  e_xc = _en.xc_energy(rho, g, vol, prob.xc)
+  rho_tot_rec = np.sum(rho_rec, axis=0)
+  g2 = np.sum(g**2, axis=-1)
+  g2h = g2.copy()
+  g2h[0, 0, 0] = 1.0
+  vh_rec = 4 * np.pi * rho_tot_rec / g2h
+  vh_rec[0, 0, 0] = 0.0
+  v_h = (vol / n) * np.real(sfft.ifftn(vh_rec, workers=_pw._WORKERS))
+  s_conj = np.exp(-1j * (g @ prob.positions.T)) @ prob.charges.astype(float)
+  ve_rec = s_conj / (g2 + 1e-10)
+  ve_rec[0, 0, 0] = 0.0
+  v_e = -4 * np.pi * np.real(sfft.ifftn(ve_rec, workers=_pw._WORKERS))
+  if rho.shape[0] == 1:
+    v_xc = (vol / n) * (4.0 / 3.0) * _en.lda_x_eps(rho)
+  else:
+    # E = vol/N sum eps(r) rho_tot, eps = 0.5 (e(2 rho_u) + e(2 rho_d))
+    eps = _en.xc_density(rho)
+    rho_tot = rho[0] + rho[1]
+    v_xc = np.empty_like(rho)
+    for s in range(2):
+      n2 = 2 * rho[s]
+      de = np.where(n2 > _en.LDA_DENS_THRESHOLD,
+                    -_en.CX / 3.0 * np.cbrt(np.maximum(n2, 1e-300))**-2, 0.0)
+      v_xc[s] = (vol / n) * (eps + rho_tot * de)
+  v = v_h[None] + v_e[None] + v_xc
+  return v, rho_rec, {"hartree": e_har, "external": e_ext, "xc": e_xc}
+
+
+def qr_backward(q, r, qbar):
+  """Wbar for L(Q(W)) with R unused; batched over leading axes."""
+  qh = np.conj(np.swapaxes(q, -1, -2))
+  x = np.triu(-qh @ qbar)
+  xh = np.conj(np.swapaxes(x, -1, -2))
+  d = np.real(np.diagonal(x, axis1=-2, axis2=-1))
+  m = x + xh - d[..., None] * np.eye(x.shape[-1])
+  y = qbar + q @ m
+  # Y R^{-H}: solve Wbar R^H = Y  <=>  R Wbar^H = Y^H
+  rh_inv_t = np.linalg.solve(r, np.conj(np.swapaxes(y, -1, -2)))
+  return np.conj(np.swapaxes(rh_inv_t, -1, -2))
+
+
+def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
+                    want_grad: bool = True):
+  """Returns (energies dict incl. 'total', grad dict or None, rho)."""
+  w = params["w_re"] + 1j * params["w_im"]
+  q, r = np.linalg.qr(w, mode="reduced")  # (s,k,g,b), (s,k,b,b)
+  ns, nk, ng, nb = q.shape
+  mask = prob.mask
+  shape = mask.shape
+  n = int(np.prod(shape))
+  vol = prob.vol
+  occ = prob.occupation
+  k_chunk = k_chunk or max(1, min(nk, 8))
+  scale = n / np.sqrt(vol)
+  rho = np.zeros((ns,) + shape)
+  for s in range(ns):
+    for k0 in range(0, nk, k_chunk):
+      k1 = min(nk, k0 + k_chunk)
+      box = _pw.expand_coefficient(q[s, k0:k1], mask)  # (kc,b,x,y,z)
+      psi = sfft.ifftn(box, axes=(-3, -2, -1), workers=_pw._WORKERS) * scale
+      rho[s] += np.einsum("kbxyz,kb->xyz", np.real(np.conj(psi) * psi),
+                          occ[s, k0:k1])
+  v, rho_rec, parts = potential_real(rho, prob)
+  gm = prob.mask_g()  # (ng,3)
+  gk2 = np.sum((gm[None] + prob.kpts[:, None, :])**2, axis=-1)  # (k,g)
+  absq = np.real(np.conj(q) * q)  # (s,k,g,b)
+  parts["kinetic"] = float(np.sum(
+    np.einsum("kg,skgb->skb", gk2, absq) * occ) / 2)
+  parts["total"] = (parts["kinetic"] + parts["hartree"] + parts["external"] +
+                    parts["xc"])
+  if not want_grad:
+    return parts, None, rho
+  qbar = np.empty_like(q)
+  for s in range(ns):
+    for k0 in range(0, nk, k_chunk):
+      k1 = min(nk, k0 + k_chunk)
+      box = _pw.expand_coefficient(q[s, k0:k1], mask)
+      psi = sfft.ifftn(box, axes=(-3, -2, -1), workers=_pw._WORKERS) * scale
+      hv = sfft.fftn(v[s] * psi, axes=(-3, -2, -1), workers=_pw._WORKERS)
+      hv = _pw.squeeze_coefficient(hv, mask)  # (kc,g,b)
+      f = occ[s, k0:k1][:, None, :]
+      qbar[s, k0:k1] = f * (2.0 / np.sqrt(vol) * hv +
+                            gk2[k0:k1, :, None] * q[s, k0:k1])
+  wbar = qr_backward(q, r, qbar)
+  return parts, {"w_re": np.real(wbar), "w_im": np.imag(wbar)}, rho
+
+
+def total_energy_literal(params, prob: Problem):
+  """The driver closure calc_ground_state_energy_all_electrons.py:119-137
+  composed from the reference's public functions (full-box layout)."""
+  c = _pw.coeff(params, prob.mask)
+  occ = prob.occupation
+  rho = _pw.density_grid(c, prob.vol, occ)
+  rho_rec = _pw.density_grid_reciprocal(c, prob.vol, occ)
+  g = prob.g_vector_grid
+  kin = _en.kinetic(g, prob.kpts, c, occ)
+  har = _en.hartree(rho_rec, g, prob.vol)
+  ext = _en.external(rho_rec, prob.positions, prob.charges, g, prob.vol)
+  xc = _en.xc_energy(rho, g, prob.vol, prob.xc)
+  return {"kinetic": kin, "hartree": har, "external": ext, "xc": xc,
+          "total": kin + har + ext + xc}

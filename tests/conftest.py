@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+  sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+  config.addinivalue_line(
+    "markers", "gpu: needs a ROCm GPU and the built libjrx_hip.so")
+  config.addinivalue_line("markers", "slow: longer CPU oracle checks")
+
+
+@pytest.fixture(scope="session")
+def golden_dir():
+  return GOLDEN
