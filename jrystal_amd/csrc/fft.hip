@@ -112,12 +112,14 @@ __device__ __forceinline__ int fftfreq_index(int j, int n) {
 
 // Scatter the mask points of one x-plane of one state into LDS, inverse
 // transform along z (only rows holding mask points) and along y, store the
-// plane.
+// plane.  NY, NZ > 0 select a compile-time schedule.
+template <int NY, int NZ, int CAP>
 __global__ __launch_bounds__(kPlaneBlock) void plane_ifft_kernel(
     const double2* __restrict__ q, double2* __restrict__ T, int64_t state0,
     BoxGeom geo, FftLen Ly, FftLen Lz) {
   extern __shared__ double2 sm[];
-  const int ny = geo.ny, nz = geo.nz, P = ny * nz;
+  const int ny = NY > 0 ? NY : geo.ny, nz = NZ > 0 ? NZ : geo.nz;
+  const int P = ny * nz;
   const int xi = blockIdx.x;
   const int c = blockIdx.y;
   const int64_t st = state0 + c;
@@ -131,8 +133,8 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_ifft_kernel(
   for (int g = g0 + threadIdx.x; g < g1; g += kPlaneBlock)
     sm[geo.g_yz[g]] = qs[g];
   __syncthreads();
-  lds_fft<true, kPlaneBlock, kPlaneCap, false>(sm, nr, nz, 1, rows, Lz);
-  lds_fft<true, kPlaneBlock, kPlaneCap, true>(sm, nz, 1, nz, nullptr, Ly);
+  fft_lines<NZ, true, kPlaneBlock, CAP, false>(sm, nr, nz, 1, rows, Lz);
+  fft_lines<NY, true, kPlaneBlock, CAP, true>(sm, nz, 1, nz, nullptr, Ly);
   double2* dst = T + ((int64_t)c * geo.nxa + xi) * P;
   for (int i = threadIdx.x; i < P; i += kPlaneBlock) dst[i] = sm[i];
 }
@@ -143,8 +145,15 @@ int launch_plane_ifft(const double2* q, double2* T, int64_t state0,
   const int P = geo.ny * geo.nz;
   size_t shm = (size_t)P * sizeof(double2);
   dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
-  hipLaunchKernelGGL(plane_ifft_kernel, grid, block, shm, s, q, T, state0, geo,
-                     Ly, Lz);
+  if (geo.ny == 64 && geo.nz == 64)
+    hipLaunchKernelGGL((plane_ifft_kernel<64, 64, 4096>), grid, block, shm, s,
+                       q, T, state0, geo, Ly, Lz);
+  else if (geo.ny == 32 && geo.nz == 32)
+    hipLaunchKernelGGL((plane_ifft_kernel<32, 32, 1024>), grid, block, shm, s,
+                       q, T, state0, geo, Ly, Lz);
+  else
+    hipLaunchKernelGGL((plane_ifft_kernel<0, 0, kPlaneCap>), grid, block, shm,
+                       s, q, T, state0, geo, Ly, Lz);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -153,33 +162,42 @@ constexpr int kXBlock = 256;
 constexpr int kXCap = 2048;
 constexpr int kXEpt = kXCap / kXBlock;
 
-__device__ __forceinline__ void xline_range(int64_t state0, int nstates,
-                                            int nspin_states, int spin,
-                                            int* cb, int* ce) {
-  int64_t b = (int64_t)spin * nspin_states - state0;
-  int64_t e = (int64_t)(spin + 1) * nspin_states - state0;
-  *cb = (int)(b < 0 ? 0 : b);
-  *ce = (int)(e > nstates ? nstates : e);
+int xlines_per_block(int nx) {
+  int LZ = kXCap / nx;
+  if (LZ > kXLines) LZ = kXLines;
+  return LZ < 1 ? 1 : LZ;
 }
 
-// One block: fixed y, z in [z0, z0+nl).  Inverse transform along x of every
-// state of the chunk; accumulates occ * |psi|^2 per spin in registers.
+int xline_groups(const BoxGeom& geo, int nstates) {
+  const int LZ = xlines_per_block(geo.nx);
+  const int tiles = geo.ny * ((geo.nz + LZ - 1) / LZ);
+  int G = (4096 + tiles - 1) / tiles;
+  if (G > kMaxXGroups) G = kMaxXGroups;
+  if (G > nstates) G = nstates;
+  return G < 1 ? 1 : G;
+}
+
+// Block (tile, group): fixed y, z in [z0, z0+nl), states of the chunk in
+// group blockIdx.y.  Inverse transform along x; accumulates occ |psi|^2 in
+// registers and adds them to the group's partial density.
+template <int NX>
 __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
-    const double2* __restrict__ T, double* __restrict__ rho,
+    const double2* __restrict__ T, double* __restrict__ rho_part,
     const double* __restrict__ occ, int64_t state0, int nstates,
-    int nspin_states, double inv_vol, BoxGeom geo, FftLen Lx, int LZ) {
+    double inv_vol, BoxGeom geo, FftLen Lx, int LZ) {
   extern __shared__ double2 sm[];
-  const int nx = geo.nx, ny = geo.ny, nz = geo.nz, P = ny * nz;
+  const int nx = NX > 0 ? NX : geo.nx;
+  const int ny = geo.ny, nz = geo.nz, P = ny * nz;
   int* px = reinterpret_cast<int*>(sm + nx * LZ);
   const int nzt = (nz + LZ - 1) / LZ;
   const int y = blockIdx.x / nzt;
   const int z0 = (blockIdx.x % nzt) * LZ;
   const int nl = min(LZ, nz - z0);
-  const int spin = blockIdx.y;
-  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
-  int cb, ce;
-  xline_range(state0, nstates, nspin_states, spin, &cb, &ce);
+  const int G = gridDim.y, gi = blockIdx.y;
+  const int per = (nstates + G - 1) / G;
+  const int cb = gi * per, ce = min(nstates, cb + per);
   if (cb >= ce) return;  // uniform over the block
+  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
   double acc[kXEpt];
 #pragma unroll
   for (int k = 0; k < kXEpt; ++k) acc[k] = 0.0;
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
       sm[px[xi] * LZ + l] = src[(int64_t)xi * P + l];
     }
     __syncthreads();
-    lds_fft<true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
+    fft_lines<NX, true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
     const double f = occ[state0 + c];
 #pragma unroll
     for (int k = 0; k < kXEpt; ++k) {
@@ -205,25 +223,20 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
     }
     __syncthreads();
   }
+  double* part = rho_part + (int64_t)gi * geo.nx * P;
 #pragma unroll
   for (int k = 0; k < kXEpt; ++k) {
     const int e = threadIdx.x + k * kXBlock;
     const int x = e / LZ, l = e % LZ;
     if (e < nxl && l < nl) {
-      const int64_t idx = (((int64_t)spin * nx + x) * ny + y) * nz + z0 + l;
-      rho[idx] += acc[k] * inv_vol;
+      const int64_t idx = ((int64_t)x * ny + y) * nz + z0 + l;
+      part[idx] += acc[k] * inv_vol;
     }
   }
 }
 
-static int xlines_per_block(int nx) {
-  int LZ = kXCap / nx;
-  if (LZ > kXLines) LZ = kXLines;
-  return LZ < 1 ? 1 : LZ;
-}
-
-int launch_xline_density(const double2* T, double* rho, const double* occ,
-                         int64_t state0, int nstates, int nspin_states,
+int launch_xline_density(const double2* T, double* rho_part, int groups,
+                         const double* occ, int64_t state0, int nstates,
                          double inv_vol, const BoxGeom& geo, const FftLen& Lx,
                          hipStream_t s) {
   if (geo.nx > kXCap) {
@@ -232,40 +245,49 @@ int launch_xline_density(const double2* T, double* rho, const double* occ,
   }
   const int LZ = xlines_per_block(geo.nx);
   const int nzt = (geo.nz + LZ - 1) / LZ;
-  const int nspin = (int)((state0 + nstates - 1) / nspin_states) + 1;
   size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
-  dim3 grid(geo.ny * nzt, nspin), block(kXBlock);
-  hipLaunchKernelGGL(xline_density_kernel, grid, block, shm, s, T, rho, occ,
-                     state0, nstates, nspin_states, inv_vol, geo, Lx, LZ);
+  dim3 grid(geo.ny * nzt, groups), block(kXBlock);
+  if (geo.nx == 64)
+    hipLaunchKernelGGL((xline_density_kernel<64>), grid, block, shm, s, T,
+                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
+  else if (geo.nx == 32)
+    hipLaunchKernelGGL((xline_density_kernel<32>), grid, block, shm, s, T,
+                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
+  else if (geo.nx == 128)
+    hipLaunchKernelGGL((xline_density_kernel<128>), grid, block, shm, s, T,
+                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
+  else
+    hipLaunchKernelGGL((xline_density_kernel<0>), grid, block, shm, s, T,
+                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
 
 // T[c] <- FFT_x(v * IFFT_x(T[c])) on the active planes, for every state.
+template <int NX>
 __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
-    double2* __restrict__ T, const double* __restrict__ v, int64_t state0,
-    int nstates, int nspin_states, BoxGeom geo, FftLen Lx, int LZ) {
+    double2* __restrict__ T, const double* __restrict__ v, int nstates,
+    BoxGeom geo, FftLen Lx, int LZ) {
   extern __shared__ double2 sm[];
-  const int nx = geo.nx, ny = geo.ny, nz = geo.nz, P = ny * nz;
+  const int nx = NX > 0 ? NX : geo.nx;
+  const int ny = geo.ny, nz = geo.nz, P = ny * nz;
   int* px = reinterpret_cast<int*>(sm + nx * LZ);
   const int nzt = (nz + LZ - 1) / LZ;
   const int y = blockIdx.x / nzt;
   const int z0 = (blockIdx.x % nzt) * LZ;
   const int nl = min(LZ, nz - z0);
-  const int spin = blockIdx.y;
-  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
-  int cb, ce;
-  xline_range(state0, nstates, nspin_states, spin, &cb, &ce);
+  const int G = gridDim.y, gi = blockIdx.y;
+  const int per = (nstates + G - 1) / G;
+  const int cb = gi * per, ce = min(nstates, cb + per);
   if (cb >= ce) return;  // uniform over the block
+  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
   const int nxl = nx * LZ;
   double vr[kXEpt];
 #pragma unroll
   for (int k = 0; k < kXEpt; ++k) {
     const int e = threadIdx.x + k * kXBlock;
     const int x = e / LZ, l = e % LZ;
-    vr[k] = (e < nxl && l < nl)
-                ? v[(((int64_t)spin * nx + x) * ny + y) * nz + z0 + l]
-                : 0.0;
+    vr[k] = (e < nxl && l < nl) ? v[((int64_t)x * ny + y) * nz + z0 + l] : 0.0;
   }
   for (int c = cb; c < ce; ++c) {
     for (int e = threadIdx.x; e < nxl; e += kXBlock) sm[e] = czero();
@@ -276,14 +298,14 @@ __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
       sm[px[xi] * LZ + l] = buf[(int64_t)xi * P + l];
     }
     __syncthreads();
-    lds_fft<true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
+    fft_lines<NX, true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
 #pragma unroll
     for (int k = 0; k < kXEpt; ++k) {
       const int e = threadIdx.x + k * kXBlock;
       if (e < nxl) sm[e] = cscale(sm[e], vr[k]);
     }
     __syncthreads();
-    lds_fft<false, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
+    fft_lines<NX, false, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
     for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
       const int l = e % nl, xi = e / nl;
       buf[(int64_t)xi * P + l] = sm[px[xi] * LZ + l];
@@ -292,32 +314,61 @@ __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
   }
 }
 
-int launch_xline_vpsi(double2* T, const double* v, int64_t state0,
-                      int nstates, int nspin_states, const BoxGeom& geo,
-                      const FftLen& Lx, hipStream_t s) {
+int launch_xline_vpsi(double2* T, const double* v, int groups, int nstates,
+                      const BoxGeom& geo, const FftLen& Lx, hipStream_t s) {
   if (geo.nx > kXCap) {
     set_error("x grid size above 2048 is not supported");
     return JRX_ERR_UNSUPPORTED;
   }
   const int LZ = xlines_per_block(geo.nx);
   const int nzt = (geo.nz + LZ - 1) / LZ;
-  const int nspin = (int)((state0 + nstates - 1) / nspin_states) + 1;
   size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
-  dim3 grid(geo.ny * nzt, nspin), block(kXBlock);
-  hipLaunchKernelGGL(xline_vpsi_kernel, grid, block, shm, s, T, v, state0,
-                     nstates, nspin_states, geo, Lx, LZ);
+  dim3 grid(geo.ny * nzt, groups), block(kXBlock);
+  if (geo.nx == 64)
+    hipLaunchKernelGGL((xline_vpsi_kernel<64>), grid, block, shm, s, T, v,
+                       nstates, geo, Lx, LZ);
+  else if (geo.nx == 32)
+    hipLaunchKernelGGL((xline_vpsi_kernel<32>), grid, block, shm, s, T, v,
+                       nstates, geo, Lx, LZ);
+  else if (geo.nx == 128)
+    hipLaunchKernelGGL((xline_vpsi_kernel<128>), grid, block, shm, s, T, v,
+                       nstates, geo, Lx, LZ);
+  else
+    hipLaunchKernelGGL((xline_vpsi_kernel<0>), grid, block, shm, s, T, v,
+                       nstates, geo, Lx, LZ);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+__global__ void reduce_groups_kernel(const double* __restrict__ part,
+                                     double* __restrict__ out, int G,
+                                     int64_t N) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double a = 0.0;
+  for (int g = 0; g < G; ++g) a += part[(int64_t)g * N + i];
+  out[i] = a;
+}
+
+int launch_reduce_groups(const double* part, double* out, int G, int64_t N,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(reduce_groups_kernel, dim3(ceil_div(N, 256)), dim3(256),
+                     0, s, part, out, G, N);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
 
 // Forward transform of one plane (y, then z on the mask rows), gather at the
 // mask points and combine with the kinetic term.
+template <int NY, int NZ, int CAP>
 __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
     const double2* __restrict__ T, const double2* __restrict__ q,
     double2* __restrict__ qbar, double* __restrict__ kin_partial,
     int64_t state0, BoxGeom geo, FftLen Ly, FftLen Lz, GatherParams gp) {
   extern __shared__ double2 sm[];
-  const int nx = geo.nx, ny = geo.ny, nz = geo.nz, P = ny * nz;
+  const int nx = geo.nx;
+  const int ny = NY > 0 ? NY : geo.ny, nz = NZ > 0 ? NZ : geo.nz;
+  const int P = ny * nz;
   const int xi = blockIdx.x;
   const int c = blockIdx.y;
   const int64_t st = state0 + c;
@@ -327,8 +378,8 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
   const int nr = geo.row_off[xi + 1] - r0;
   const short* rows = geo.rows + r0;
   __syncthreads();
-  lds_fft<false, kPlaneBlock, kPlaneCap, true>(sm, nz, 1, nz, nullptr, Ly);
-  lds_fft<false, kPlaneBlock, kPlaneCap, false>(sm, nr, nz, 1, rows, Lz);
+  fft_lines<NY, false, kPlaneBlock, CAP, true>(sm, nz, 1, nz, nullptr, Ly);
+  fft_lines<NZ, false, kPlaneBlock, CAP, false>(sm, nr, nz, 1, rows, Lz);
   const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
   const int k = (int)((st / gp.nb) % gp.nk);
   const double f = gp.occ[st];
@@ -369,8 +420,18 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
   size_t shm = (size_t)P * sizeof(double2);
   if (shm < kPlaneBlock * sizeof(double)) shm = kPlaneBlock * sizeof(double);
   dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
-  hipLaunchKernelGGL(plane_fft_gather_kernel, grid, block, shm, s, T, q, qbar,
-                     kin_partial, state0, geo, Ly, Lz, gp);
+  if (geo.ny == 64 && geo.nz == 64)
+    hipLaunchKernelGGL((plane_fft_gather_kernel<64, 64, 4096>), grid, block,
+                       shm, s, T, q, qbar, kin_partial, state0, geo, Ly, Lz,
+                       gp);
+  else if (geo.ny == 32 && geo.nz == 32)
+    hipLaunchKernelGGL((plane_fft_gather_kernel<32, 32, 1024>), grid, block,
+                       shm, s, T, q, qbar, kin_partial, state0, geo, Ly, Lz,
+                       gp);
+  else
+    hipLaunchKernelGGL((plane_fft_gather_kernel<0, 0, kPlaneCap>), grid,
+                       block, shm, s, T, q, qbar, kin_partial, state0, geo,
+                       Ly, Lz, gp);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

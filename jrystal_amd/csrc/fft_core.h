@@ -9,6 +9,8 @@
 // Reads of a stage complete (barrier) before its writes, so one buffer
 // suffices.  All transforms are unnormalised; INV selects e^{+i}.
 #pragma once
+#include <type_traits>
+
 #include "common.h"
 
 namespace jrx {
@@ -199,6 +201,55 @@ __device__ void lds_fft(double2* buf, int nlines, int ls, int es,
     }
     p *= R;
   }
+}
+
+// ------------------------------------------------- compile-time schedules --
+// Radix schedules of the lengths with specialised kernels; the stage loop is
+// unrolled and every index constant folds.
+template <int... Rs>
+struct RadixList {};
+template <int N>
+struct RadixOf {
+  using type = void;  // no compile-time schedule: use lds_fft
+};
+template <> struct RadixOf<16> { using type = RadixList<8, 2>; };
+template <> struct RadixOf<32> { using type = RadixList<8, 4>; };
+template <> struct RadixOf<64> { using type = RadixList<8, 8>; };
+template <> struct RadixOf<128> { using type = RadixList<8, 4, 4>; };
+
+template <bool INV, int BLOCK, int CAP, bool LINE_FAST, int N, int P, int R,
+          int... Rest>
+__device__ __forceinline__ void ct_stages(double2* buf, int nlines, int ls,
+                                          int es, const short* lmap,
+                                          const double2* __restrict__ tw) {
+  fft_stage<R, INV, BLOCK, CAP, LINE_FAST>(buf, N, nlines, ls, es, lmap, P,
+                                           tw);
+  if constexpr (sizeof...(Rest) > 0)
+    ct_stages<INV, BLOCK, CAP, LINE_FAST, N, P * R, Rest...>(buf, nlines, ls,
+                                                             es, lmap, tw);
+}
+
+template <bool INV, int BLOCK, int CAP, bool LINE_FAST, int N, int... Rs>
+__device__ __forceinline__ void ct_fft_impl(RadixList<Rs...>, double2* buf,
+                                            int nlines, int ls, int es,
+                                            const short* lmap,
+                                            const double2* tw) {
+  ct_stages<INV, BLOCK, CAP, LINE_FAST, N, 1, Rs...>(buf, nlines, ls, es,
+                                                     lmap, tw);
+}
+
+// Transform of length N (compile-time schedule when N > 0 has one, the
+// runtime schedule of L otherwise).
+template <int N, bool INV, int BLOCK, int CAP, bool LINE_FAST>
+__device__ __forceinline__ void fft_lines(double2* buf, int nlines, int ls,
+                                          int es, const short* lmap,
+                                          const FftLen& L) {
+  if constexpr (N > 0 && !std::is_same<typename RadixOf<N>::type, void>::value)
+    ct_fft_impl<INV, BLOCK, CAP, LINE_FAST, N>(typename RadixOf<N>::type{},
+                                               buf, nlines, ls, es, lmap,
+                                               L.tw);
+  else
+    lds_fft<INV, BLOCK, CAP, LINE_FAST>(buf, nlines, ls, es, lmap, L);
 }
 
 }  // namespace jrx

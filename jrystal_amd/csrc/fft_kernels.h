@@ -34,17 +34,26 @@ int launch_plane_ifft(const double2* q, double2* T, int64_t state0,
                       int nstates, const BoxGeom& geo, const FftLen& Ly,
                       const FftLen& Lz, hipStream_t s);
 
-// rho_s(x,y,z) += sum_c occ[c] |IFFT_x T[c]|^2 / vol over the states
-// c of the chunk that belong to spin s (state / nspin_states == s).
-int launch_xline_density(const double2* T, double* rho, const double* occ,
-                         int64_t state0, int nstates, int nspin_states,
+// Groups of states per x-line launch (parallelism across states).
+constexpr int kMaxXGroups = 32;
+int xlines_per_block(int nx);
+int xline_groups(const BoxGeom& geo, int nstates);
+
+// rho_part[g](x,y,z) += sum_{c in group g} occ[state0+c] |IFFT_x T[c]|^2 /
+// vol.  All states of the chunk belong to one spin.
+int launch_xline_density(const double2* T, double* rho_part, int groups,
+                         const double* occ, int64_t state0, int nstates,
                          double inv_vol, const BoxGeom& geo, const FftLen& Lx,
                          hipStream_t s);
 
-// T[c] <- FFT_x( v_s * IFFT_x(T[c]) ) restricted to the active planes.
-int launch_xline_vpsi(double2* T, const double* v, int64_t state0,
-                      int nstates, int nspin_states, const BoxGeom& geo,
-                      const FftLen& Lx, hipStream_t s);
+// out[i] = sum_g part[g*N + i] (fixed order).
+int launch_reduce_groups(const double* part, double* out, int G, int64_t N,
+                         hipStream_t s);
+
+// T[c] <- FFT_x( v * IFFT_x(T[c]) ) restricted to the active planes; v is
+// the potential of the chunk's spin.
+int launch_xline_vpsi(double2* T, const double* v, int groups, int nstates,
+                      const BoxGeom& geo, const FftLen& Lx, hipStream_t s);
 
 // qbar[state][g] = f (2/vol * FFT_yz(T)[g] + |G+k|^2 q[state][g]) and the
 // per-(state, plane) kinetic partials f |G+k|^2 |q|^2 (summed later).
@@ -64,6 +73,6 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
 // Largest plane (ny*nz) the fused plane kernels hold in LDS.
 constexpr int kPlaneCap = 6400;
 // Lines per block of the x-line kernels.
-constexpr int kXLines = 16;
+constexpr int kXLines = 32;
 
 }  // namespace jrx

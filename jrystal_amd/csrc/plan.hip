@@ -77,7 +77,8 @@ struct jrx_plan {
   short* d_g_m = nullptr;
   double* d_kpts = nullptr;
   double* d_occ = nullptr;
-  int chunk = 0;
+  int chunk = 0;   // states per FFT chunk (never straddles spins)
+  int groups = 1;  // state groups of the x-line kernels
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
   // timing
@@ -85,7 +86,7 @@ struct jrx_plan {
   std::vector<hipEvent_t> events;
   std::vector<std::pair<int, int>> records;  // (class, event pair index)
   // workspace offsets (bytes)
-  size_t o_q, o_qbar, o_T, o_rho, o_v, o_box, o_S, o_R1, o_R2, o_Ri, o_A,
+  size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_S, o_R1, o_R2, o_Ri, o_A,
       o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, total;
 };
 
@@ -236,10 +237,9 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
       &max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
   const int nb = d->nb;
   const size_t lds_plane = (size_t)ny * nz * sizeof(double2);
-  const size_t lds_qr = ((size_t)kQrRows * (nb + 1) + (size_t)nb * (kQrRows + 1)) *
-                        sizeof(double2);
-  const size_t lds_gram = ((size_t)kQrRows * (nb + 1) + (size_t)nb * nb) *
-                          sizeof(double2);
+  const size_t nbp = 16 * (size_t)((nb + 15) / 16);
+  const size_t lds_qr = 2 * nbp * nbp * sizeof(double2);
+  const size_t lds_gram = nbp * nbp * sizeof(double2);
   const size_t lds_chol = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
   size_t need = lds_plane;
   for (size_t v : {lds_qr, lds_gram, lds_chol}) need = v > need ? v : need;
@@ -257,9 +257,11 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     chunk = (int64_t)(budget / per_state);
     if (chunk < 1) chunk = 1;
   }
-  if (chunk > p->S) chunk = p->S;
+  const int64_t spin_states = (int64_t)d->nk * d->nb;
+  if (chunk > spin_states) chunk = spin_states;
   if (chunk > 65535) chunk = 65535;
   p->chunk = (int)chunk;
+  p->groups = xline_groups(p->geo, p->chunk);
   p->gsw = gram_split(p->P, ng);
   p->gsq = p->gsw;
   const int64_t kin_gather = p->S * p->geo.nxa;
@@ -278,6 +280,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_qbar = take((size_t)p->S * ng * sizeof(double2));
   p->o_T = take((size_t)p->chunk * per_state);
   p->o_rho = take((size_t)p->ns * p->N * sizeof(double));
+  p->o_rpart = take((size_t)p->groups * p->N * sizeof(double));
   p->o_v = take((size_t)p->ns * p->N * sizeof(double));
   p->o_box = take((size_t)p->N * sizeof(double2));
   p->o_S = take(mat);
@@ -286,7 +289,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_Ri = take(mat);
   p->o_A = take(mat);
   p->o_Bm = take(mat);
-  p->o_part = take((size_t)p->P * p->gsw.nchunk * nb * nb * sizeof(double2));
+  p->o_part = take((size_t)p->P * p->gsw.nchunk * 4 * nb * nb * sizeof(double2));
   p->o_kin = take((size_t)p->kin_parts_max * sizeof(double));
   p->o_eh = take((size_t)p->rec_parts * sizeof(double));
   p->o_ee = take((size_t)p->rec_parts * sizeof(double));
@@ -428,6 +431,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   double2* qbar = cptr(p->o_qbar);
   double2* T = cptr(p->o_T);
   double* rho = dptr(p->o_rho);
+  double* rpart = dptr(p->o_rpart);
   double* v = dptr(p->o_v);
   double2* box = cptr(p->o_box);
   double2* Sm = cptr(p->o_S);
@@ -453,21 +457,29 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   } while (0)
   // --- QR (CholeskyQR2) ----------------------------------------------------
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, p->gsw, s));
-  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsw.nchunk, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s));
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s));
-  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsw.nchunk, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));
   JRX_T(JRX_K_QR_APPLY, launch_apply_fwd(w_re, w_im, Ri, q, P, ng, nb, s));
   // --- density -------------------------------------------------------------
-  JRX_T(JRX_K_OTHER, launch_fill(rho, 0.0, (int64_t)p->ns * p->N, s));
-  const int nspin_states = p->nk * nb;
-  for (int64_t s0 = 0; s0 < S; s0 += p->chunk) {
-    const int nst = (int)((S - s0) < p->chunk ? (S - s0) : p->chunk);
-    JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, T, s0, nst, p->geo, p->Ly, p->Lz, s));
-    JRX_T(JRX_K_XLINE_DENSITY, launch_xline_density(T, rho, p->d_occ, s0, nst, nspin_states,
-                                 1.0 / p->vol, p->geo, p->Lx, s));
+  const int64_t SS = (int64_t)p->nk * nb;  // states per spin
+  for (int sp = 0; sp < p->ns; ++sp) {
+    JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
+    for (int64_t s0 = 0; s0 < SS; s0 += p->chunk) {
+      const int nst = (int)((SS - s0) < p->chunk ? (SS - s0) : p->chunk);
+      const int64_t st0 = sp * SS + s0;
+      JRX_T(JRX_K_PLANE_IFFT,
+            launch_plane_ifft(q, T, st0, nst, p->geo, p->Ly, p->Lz, s));
+      JRX_T(JRX_K_XLINE_DENSITY,
+            launch_xline_density(T, rpart, xline_groups(p->geo, nst),
+                                 p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
+                                 p->Lx, s));
+    }
+    JRX_T(JRX_K_XLINE_DENSITY,
+          launch_reduce_groups(rpart, rho + sp * p->N, p->groups, p->N, s));
   }
   // --- reciprocal energies and potential ----------------------------------
   JRX_T(JRX_K_POTENTIAL, launch_rho_to_complex(rho, box, p->ns, p->N, s));
@@ -488,17 +500,24 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
     gp.nk = p->nk;
     gp.nb = nb;
     gp.two_over_vol = 2.0 / p->vol;
-    for (int64_t s0 = 0; s0 < S; s0 += p->chunk) {
-      const int nst = (int)((S - s0) < p->chunk ? (S - s0) : p->chunk);
-      JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, T, s0, nst, p->geo, p->Ly, p->Lz, s));
-      JRX_T(JRX_K_XLINE_VPSI, launch_xline_vpsi(T, v, s0, nst, nspin_states, p->geo, p->Lx, s));
-      JRX_T(JRX_K_PLANE_FFT_GATHER, launch_plane_fft_gather(T, q, qbar, kin, s0, nst, p->geo, p->Ly,
-                                      p->Lz, gp, s));
+    for (int sp = 0; sp < p->ns; ++sp) {
+      for (int64_t s0 = 0; s0 < SS; s0 += p->chunk) {
+        const int nst = (int)((SS - s0) < p->chunk ? (SS - s0) : p->chunk);
+        const int64_t st0 = sp * SS + s0;
+        JRX_T(JRX_K_PLANE_IFFT,
+              launch_plane_ifft(q, T, st0, nst, p->geo, p->Ly, p->Lz, s));
+        JRX_T(JRX_K_XLINE_VPSI,
+              launch_xline_vpsi(T, v + sp * p->N, xline_groups(p->geo, nst),
+                                nst, p->geo, p->Lx, s));
+        JRX_T(JRX_K_PLANE_FFT_GATHER,
+              launch_plane_fft_gather(T, q, qbar, kin, st0, nst, p->geo,
+                                      p->Ly, p->Lz, gp, s));
+      }
     }
     nkin = (int)(S * p->geo.nxa);
     // --- QR adjoint ----------------------------------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
-    JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsq.nchunk, nb, s));
+    JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsq.nchunk, nb, s));
     JRX_T(JRX_K_QR_SMALL, launch_qr_bwd_small(Sm, Ri, Am, Bm, P, nb, s));
     JRX_T(JRX_K_QR_BWD_APPLY, launch_apply_bwd(qbar, q, Am, Bm, dw_re, dw_im, P, ng, nb, s));
   } else {

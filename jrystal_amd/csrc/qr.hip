@@ -27,151 +27,217 @@ GramSplit gram_split(int64_t P, int ng) {
   return sp;
 }
 
-constexpr int kMaxOut = kMaxBands * kMaxBands / kQrBlock;    // 16
-constexpr int kMaxTile = kQrRows * kMaxBands / kQrBlock;     // 16
 
-template <bool PRE>
-__global__ __launch_bounds__(kQrBlock) void gram_gmajor_kernel(
+// ------------------------------------------------------------- FP64 MFMA --
+// v_mfma_f64_16x16x4f64 on one wave: D(16x16) += A(16x4) B(4x16), lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; accumulator register r of lane
+// l holds D[4r + l/16][l%16].  Complex products are four real products.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ d4 d4zero() { return d4{0.0, 0.0, 0.0, 0.0}; }
+
+// Small nb x nb matrix into LDS, zero-padded to NBP x NBP.
+__device__ __forceinline__ void load_small(double2* dst, const double2* src,
+                                           int nb, int NBP) {
+  for (int e = threadIdx.x; e < NBP * NBP; e += kQrBlock) {
+    const int i = e / NBP, j = e - i * NBP;
+    dst[e] = (i < nb && j < nb) ? src[i * nb + j] : czero();
+  }
+}
+
+// Per-wave partial Gram sums over the chunk's rows:
+//   S = A^H A with A = W (PRE = false) or A = W pre (PRE = true).
+// Waves take 16-row slabs in turn; only tile pairs I <= J are accumulated
+// and the full Hermitian matrix is written.
+template <int T, bool PRE>
+__global__ __launch_bounds__(kQrBlock) void gram_w_kernel(
     const double* __restrict__ wr, const double* __restrict__ wi,
     const double2* __restrict__ pre, double2* __restrict__ partial, int ng,
     int nb, int rows_per_chunk, int nchunk) {
   extern __shared__ double2 sm[];
-  const int ld = nb + 1;
-  double2* tile = sm;                   // [kQrRows][nb+1]
-  double2* pm = sm + kQrRows * ld;      // [nb][nb]
+  constexpr int NBP = 16 * T;
+  constexpr int NP = T * (T + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   const int64_t p = blockIdx.y;
   const int chunk = blockIdx.x;
-  const int tid = threadIdx.x;
-  if (PRE)
-    for (int i = tid; i < nb * nb; i += kQrBlock) pm[i] = pre[p * nb * nb + i];
-  double2 acc[kMaxOut];
+  if (PRE) load_small(sm, pre + p * nb * nb, nb, NBP);
+  __syncthreads();
+  d4 sre[NP], sim[NP];
 #pragma unroll
-  for (int k = 0; k < kMaxOut; ++k) acc[k] = czero();
-  const int nout = nb * nb;
+  for (int i = 0; i < NP; ++i) { sre[i] = d4zero(); sim[i] = d4zero(); }
   const int gb = chunk * rows_per_chunk;
   const int ge = min(ng, gb + rows_per_chunk);
-  for (int gt = gb; gt < ge; gt += kQrRows) {
-    const int nr = min(kQrRows, ge - gt);
-    __syncthreads();
-    const double* r0 = wr + (p * ng + gt) * nb;
-    const double* i0 = wi + (p * ng + gt) * nb;
-    for (int e = tid; e < nr * nb; e += kQrBlock) {
-      const int r = e / nb, c = e - r * nb;
-      tile[r * ld + c] = make_double2(r0[e], i0[e]);
-    }
-    __syncthreads();
+  const double* Wr = wr + p * ng * nb;
+  const double* Wi = wi + p * ng * nb;
+  const int nm = (nb + 3) / 4;
+  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
+    double xr[T][4], xi[T][4];
     if (PRE) {
-      double2 tmp[kMaxTile];
+      d4 are[T], aim[T];
 #pragma unroll
-      for (int k = 0; k < kMaxTile; ++k) {
-        const int e = tid + k * kQrBlock;
-        double2 a = czero();
-        if (e < nr * nb) {
-          const int r = e / nb, c = e - r * nb;
-          for (int i = 0; i <= c; ++i)
-            a = cadd(a, cmul(tile[r * ld + i], pm[i * nb + c]));
-        }
-        tmp[k] = a;
-      }
-      __syncthreads();
+      for (int J = 0; J < T; ++J) { are[J] = d4zero(); aim[J] = d4zero(); }
+      const int row = g0 + li;
+      for (int m = 0; m < nm; ++m) {
+        const int col = 4 * m + lk;
+        const bool ok = row < ge && col < nb;
+        const double a_r = ok ? Wr[(int64_t)row * nb + col] : 0.0;
+        const double a_i = ok ? Wi[(int64_t)row * nb + col] : 0.0;
 #pragma unroll
-      for (int k = 0; k < kMaxTile; ++k) {
-        const int e = tid + k * kQrBlock;
-        if (e < nr * nb) {
-          const int r = e / nb, c = e - r * nb;
-          tile[r * ld + c] = tmp[k];
+        for (int J = 0; J < T; ++J) {
+          const double2 rv = sm[(4 * m + lk) * NBP + 16 * J + li];
+          are[J] = mfma4(a_r, rv.x, are[J]);
+          are[J] = mfma4(-a_i, rv.y, are[J]);
+          aim[J] = mfma4(a_r, rv.y, aim[J]);
+          aim[J] = mfma4(a_i, rv.x, aim[J]);
         }
       }
-      __syncthreads();
+#pragma unroll
+      for (int J = 0; J < T; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { xr[J][r] = are[J][r]; xi[J][r] = aim[J][r]; }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = g0 + 4 * r + lk;
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+          const int col = 16 * J + li;
+          const bool ok = row < ge && col < nb;
+          xr[J][r] = ok ? Wr[(int64_t)row * nb + col] : 0.0;
+          xi[J][r] = ok ? Wi[(int64_t)row * nb + col] : 0.0;
+        }
+      }
     }
-    for (int r = 0; r < nr; ++r) {
 #pragma unroll
-      for (int k = 0; k < kMaxOut; ++k) {
-        const int o = tid + k * kQrBlock;
-        if (o < nout) {
-          const int i = o / nb, j = o - i * nb;
-          acc[k] = cadd(acc[k], cmulc(tile[r * ld + i], tile[r * ld + j]));
+    for (int r = 0; r < 4; ++r) {
+      int pi = 0;
+#pragma unroll
+      for (int I = 0; I < T; ++I)
+#pragma unroll
+        for (int J = I; J < T; ++J, ++pi) {
+          sre[pi] = mfma4(xr[I][r], xr[J][r], sre[pi]);
+          sre[pi] = mfma4(xi[I][r], xi[J][r], sre[pi]);
+          sim[pi] = mfma4(xr[I][r], xi[J][r], sim[pi]);
+          sim[pi] = mfma4(-xi[I][r], xr[J][r], sim[pi]);
         }
-      }
     }
   }
-  double2* out = partial + (p * nchunk + chunk) * nout;
+  double2* out = partial + ((p * nchunk + chunk) * 4 + wave) * nb * nb;
+  int pi = 0;
 #pragma unroll
-  for (int k = 0; k < kMaxOut; ++k) {
-    const int o = tid + k * kQrBlock;
-    if (o < nout) out[o] = acc[k];
-  }
+  for (int I = 0; I < T; ++I)
+#pragma unroll
+    for (int J = I; J < T; ++J, ++pi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * r + lk, j = 16 * J + li;
+        if (i < nb && j < nb) {
+          out[i * nb + j] = make_double2(sre[pi][r], sim[pi][r]);
+          if (I < J) out[j * nb + i] = make_double2(sre[pi][r], -sim[pi][r]);
+        }
+      }
 }
 
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
                        const GramSplit& sp, hipStream_t s) {
-  size_t shm = (size_t)kQrRows * (nb + 1) * sizeof(double2);
-  if (pre) shm += (size_t)nb * nb * sizeof(double2);
+  const int T = (nb + 15) / 16;
+  size_t shm = pre ? (size_t)(16 * T) * (16 * T) * sizeof(double2) : 16;
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
-  if (pre)
-    hipLaunchKernelGGL((gram_gmajor_kernel<true>), grid, block, shm, s, wr, wi,
-                       pre, partial, ng, nb, sp.rows_per_chunk, sp.nchunk);
-  else
-    hipLaunchKernelGGL((gram_gmajor_kernel<false>), grid, block, shm, s, wr,
-                       wi, pre, partial, ng, nb, sp.rows_per_chunk, sp.nchunk);
+#define JRX_GRAM_W(TT)                                                        \
+  if (pre)                                                                    \
+    hipLaunchKernelGGL((gram_w_kernel<TT, true>), grid, block, shm, s, wr, wi, \
+                       pre, partial, ng, nb, sp.rows_per_chunk, sp.nchunk);   \
+  else                                                                        \
+    hipLaunchKernelGGL((gram_w_kernel<TT, false>), grid, block, shm, s, wr,   \
+                       wi, pre, partial, ng, nb, sp.rows_per_chunk,           \
+                       sp.nchunk);
+  switch (T) {
+    case 1: JRX_GRAM_W(1) break;
+    case 2: JRX_GRAM_W(2) break;
+    case 3: JRX_GRAM_W(3) break;
+    case 4: JRX_GRAM_W(4) break;
+    default:
+      set_error("nb above 64 is not supported by this build");
+      return JRX_ERR_UNSUPPORTED;
+  }
+#undef JRX_GRAM_W
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
 
-__global__ __launch_bounds__(kQrBlock) void gram_bmajor_kernel(
+// Per-wave partial sums of G = Q^H Qbar over the chunk (band-major inputs).
+template <int T>
+__global__ __launch_bounds__(kQrBlock) void gram_q_kernel(
     const double2* __restrict__ q, const double2* __restrict__ qb,
     double2* __restrict__ partial, int ng, int nb, int rows_per_chunk,
     int nchunk) {
-  extern __shared__ double2 sm[];
-  const int ld = kQrRows + 1;
-  double2* tq = sm;              // [nb][kQrRows+1]
-  double2* tb = sm + nb * ld;    // [nb][kQrRows+1]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   const int64_t p = blockIdx.y;
   const int chunk = blockIdx.x;
-  const int tid = threadIdx.x;
-  double2 acc[kMaxOut];
+  d4 gre[T][T], gim[T][T];
 #pragma unroll
-  for (int k = 0; k < kMaxOut; ++k) acc[k] = czero();
-  const int nout = nb * nb;
+  for (int I = 0; I < T; ++I)
+#pragma unroll
+    for (int J = 0; J < T; ++J) { gre[I][J] = d4zero(); gim[I][J] = d4zero(); }
   const int gb = chunk * rows_per_chunk;
   const int ge = min(ng, gb + rows_per_chunk);
-  for (int gt = gb; gt < ge; gt += kQrRows) {
-    const int nr = min(kQrRows, ge - gt);
-    __syncthreads();
-    for (int e = tid; e < nb * nr; e += kQrBlock) {
-      const int i = e / nr, r = e - i * nr;
-      const int64_t src = (p * nb + i) * ng + gt + r;
-      tq[i * ld + r] = q[src];
-      tb[i * ld + r] = qb[src];
-    }
-    __syncthreads();
-    for (int r = 0; r < nr; ++r) {
+  const double2* Q = q + p * nb * ng;
+  const double2* B = qb + p * nb * ng;
+  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
 #pragma unroll
-      for (int k = 0; k < kMaxOut; ++k) {
-        const int o = tid + k * kQrBlock;
-        if (o < nout) {
-          const int i = o / nb, j = o - i * nb;
-          acc[k] = cadd(acc[k], cmulc(tq[i * ld + r], tb[j * ld + r]));
-        }
+    for (int r = 0; r < 4; ++r) {
+      const int g = g0 + 4 * r + lk;
+      double2 a[T], b[T];
+#pragma unroll
+      for (int I = 0; I < T; ++I) {
+        const int band = 16 * I + li;
+        const bool ok = g < ge && band < nb;
+        a[I] = ok ? Q[(int64_t)band * ng + g] : czero();
+        b[I] = ok ? B[(int64_t)band * ng + g] : czero();
       }
+#pragma unroll
+      for (int I = 0; I < T; ++I)
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+          gre[I][J] = mfma4(a[I].x, b[J].x, gre[I][J]);
+          gre[I][J] = mfma4(a[I].y, b[J].y, gre[I][J]);
+          gim[I][J] = mfma4(a[I].x, b[J].y, gim[I][J]);
+          gim[I][J] = mfma4(-a[I].y, b[J].x, gim[I][J]);
+        }
     }
   }
-  double2* out = partial + (p * nchunk + chunk) * nout;
+  double2* out = partial + ((p * nchunk + chunk) * 4 + wave) * nb * nb;
 #pragma unroll
-  for (int k = 0; k < kMaxOut; ++k) {
-    const int o = tid + k * kQrBlock;
-    if (o < nout) out[o] = acc[k];
-  }
+  for (int I = 0; I < T; ++I)
+#pragma unroll
+    for (int J = 0; J < T; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * r + lk, j = 16 * J + li;
+        if (i < nb && j < nb) out[i * nb + j] = make_double2(gre[I][J][r], gim[I][J][r]);
+      }
 }
 
 int launch_gram_bmajor(const double2* q, const double2* qb, double2* partial,
                        int64_t P, int ng, int nb, const GramSplit& sp,
                        hipStream_t s) {
-  size_t shm = 2 * (size_t)nb * (kQrRows + 1) * sizeof(double2);
+  const int T = (nb + 15) / 16;
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
-  hipLaunchKernelGGL(gram_bmajor_kernel, grid, block, shm, s, q, qb, partial,
-                     ng, nb, sp.rows_per_chunk, sp.nchunk);
+  switch (T) {
+    case 1: hipLaunchKernelGGL((gram_q_kernel<1>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 2: hipLaunchKernelGGL((gram_q_kernel<2>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 3: hipLaunchKernelGGL((gram_q_kernel<3>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 4: hipLaunchKernelGGL((gram_q_kernel<4>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    default:
+      set_error("nb above 64 is not supported by this build");
+      return JRX_ERR_UNSUPPORTED;
+  }
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -278,53 +344,6 @@ int launch_trmm_uu(const double2* a, const double2* b, double2* c, int64_t P,
   return JRX_OK;
 }
 
-__global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
-    const double* __restrict__ wr, const double* __restrict__ wi,
-    const double2* __restrict__ rinv, double2* __restrict__ q, int ng,
-    int nb) {
-  extern __shared__ double2 sm[];
-  const int ldw = nb + 1, ldo = kQrRows + 1;
-  double2* tile = sm;                  // [kQrRows][nb+1]
-  double2* otile = sm + kQrRows * ldw;  // [nb][kQrRows+1]
-  const int64_t p = blockIdx.y;
-  const int gt = blockIdx.x * kQrRows;
-  const int nr = min(kQrRows, ng - gt);
-  const int tid = threadIdx.x;
-  const double* r0 = wr + (p * ng + gt) * nb;
-  const double* i0 = wi + (p * ng + gt) * nb;
-  for (int e = tid; e < nr * nb; e += kQrBlock) {
-    const int r = e / nb, c = e - r * nb;
-    tile[r * ldw + c] = make_double2(r0[e], i0[e]);
-  }
-  __syncthreads();
-  const double2* R = rinv + p * nb * nb;
-  const int nrg = kQrBlock / nb;
-  const int j = tid % nb, rg = tid / nb;
-  if (rg < nrg) {
-    for (int r = rg; r < nr; r += nrg) {
-      double2 acc = czero();
-      for (int i = 0; i <= j; ++i) acc = cadd(acc, cmul(tile[r * ldw + i], R[i * nb + j]));
-      otile[j * ldo + r] = acc;
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < nb * nr; e += kQrBlock) {
-    const int b = e / nr, r = e - b * nr;
-    q[(p * nb + b) * ng + gt + r] = otile[b * ldo + r];
-  }
-}
-
-int launch_apply_fwd(const double* wr, const double* wi, const double2* rinv,
-                     double2* q, int64_t P, int ng, int nb, hipStream_t s) {
-  size_t shm = ((size_t)kQrRows * (nb + 1) + (size_t)nb * (kQrRows + 1)) *
-               sizeof(double2);
-  dim3 grid(ceil_div(ng, kQrRows), (unsigned)P), block(kQrBlock);
-  hipLaunchKernelGGL(apply_fwd_kernel, grid, block, shm, s, wr, wi, rinv, q,
-                     ng, nb);
-  JRX_LAUNCH_CHECK();
-  return JRX_OK;
-}
-
 __global__ __launch_bounds__(kQrBlock) void qr_bwd_small_kernel(
     const double2* __restrict__ G, const double2* __restrict__ rinv,
     double2* __restrict__ A, double2* __restrict__ Bm, int nb) {
@@ -367,47 +386,153 @@ int launch_qr_bwd_small(const double2* G, const double2* rinv, double2* A,
   return JRX_OK;
 }
 
+
+constexpr int kApplyRows = 256;  // rows per block of the apply kernels
+
+// Q[p][b][g] = sum_i W[p][g][i] rinv[p][i][b], computed as Q^T = R^T W^T so
+// that each accumulator row is 16 consecutive g of one band.
+template <int T>
+__global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
+    const double* __restrict__ wr, const double* __restrict__ wi,
+    const double2* __restrict__ rinv, double2* __restrict__ q, int ng,
+    int nb) {
+  extern __shared__ double2 sm[];
+  constexpr int NBP = 16 * T;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t p = blockIdx.y;
+  load_small(sm, rinv + p * nb * nb, nb, NBP);
+  __syncthreads();
+  const double* Wr = wr + p * ng * nb;
+  const double* Wi = wi + p * ng * nb;
+  double2* Q = q + p * nb * ng;
+  const int gb = blockIdx.x * kApplyRows;
+  const int ge = min(ng, gb + kApplyRows);
+  const int nm = (nb + 3) / 4;
+  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
+    d4 qre[T], qim[T];
+#pragma unroll
+    for (int J = 0; J < T; ++J) { qre[J] = d4zero(); qim[J] = d4zero(); }
+    const int row = g0 + li;
+    for (int m = 0; m < nm; ++m) {
+      const int col = 4 * m + lk;
+      const bool ok = row < ge && col < nb;
+      const double a_r = ok ? Wr[(int64_t)row * nb + col] : 0.0;
+      const double a_i = ok ? Wi[(int64_t)row * nb + col] : 0.0;
+#pragma unroll
+      for (int J = 0; J < T; ++J) {
+        const double2 rv = sm[(4 * m + lk) * NBP + 16 * J + li];
+        qre[J] = mfma4(rv.x, a_r, qre[J]);
+        qre[J] = mfma4(-rv.y, a_i, qre[J]);
+        qim[J] = mfma4(rv.x, a_i, qim[J]);
+        qim[J] = mfma4(rv.y, a_r, qim[J]);
+      }
+    }
+    const int g = g0 + li;
+#pragma unroll
+    for (int J = 0; J < T; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = 16 * J + 4 * r + lk;
+        if (b < nb && g < ge) Q[(int64_t)b * ng + g] = make_double2(qre[J][r], qim[J][r]);
+      }
+  }
+}
+
+int launch_apply_fwd(const double* wr, const double* wi, const double2* rinv,
+                     double2* q, int64_t P, int ng, int nb, hipStream_t s) {
+  const int T = (nb + 15) / 16;
+  size_t shm = (size_t)(16 * T) * (16 * T) * sizeof(double2);
+  dim3 grid(ceil_div(ng, kApplyRows), (unsigned)P), block(kQrBlock);
+  switch (T) {
+    case 1: hipLaunchKernelGGL((apply_fwd_kernel<1>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
+    case 2: hipLaunchKernelGGL((apply_fwd_kernel<2>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
+    case 3: hipLaunchKernelGGL((apply_fwd_kernel<3>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
+    case 4: hipLaunchKernelGGL((apply_fwd_kernel<4>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
+    default:
+      set_error("nb above 64 is not supported by this build");
+      return JRX_ERR_UNSUPPORTED;
+  }
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+// Wbar[p][g][j] = sum_i Qbar[p][i][g] A[i][j] + Q[p][i][g] Bm[i][j], planar
+// g-major outputs.
+template <int T>
 __global__ __launch_bounds__(kQrBlock) void apply_bwd_kernel(
     const double2* __restrict__ qbar, const double2* __restrict__ q,
     const double2* __restrict__ A, const double2* __restrict__ Bm,
     double* __restrict__ dwr, double* __restrict__ dwi, int ng, int nb) {
   extern __shared__ double2 sm[];
-  const int ld = kQrRows + 1;
-  double2* tb = sm;            // [nb][kQrRows+1] Qbar tile
-  double2* tq = sm + nb * ld;  // [nb][kQrRows+1] Q tile
+  constexpr int NBP = 16 * T;
+  double2* sa = sm;
+  double2* sb = sm + NBP * NBP;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   const int64_t p = blockIdx.y;
-  const int gt = blockIdx.x * kQrRows;
-  const int nr = min(kQrRows, ng - gt);
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nr; e += kQrBlock) {
-    const int i = e / nr, r = e - i * nr;
-    const int64_t src = (p * nb + i) * ng + gt + r;
-    tb[i * ld + r] = qbar[src];
-    tq[i * ld + r] = q[src];
-  }
+  load_small(sa, A + p * nb * nb, nb, NBP);
+  load_small(sb, Bm + p * nb * nb, nb, NBP);
   __syncthreads();
-  const double2* Ap = A + p * nb * nb;
-  const double2* Bp = Bm + p * nb * nb;
-  const int nrg = kQrBlock / nb;
-  const int j = tid % nb, rg = tid / nb;
-  if (rg >= nrg) return;
-  for (int r = rg; r < nr; r += nrg) {
-    double2 acc = czero();
-    for (int i = j; i < nb; ++i) acc = cadd(acc, cmul(tb[i * ld + r], Ap[i * nb + j]));
-    for (int i = 0; i < nb; ++i) acc = cadd(acc, cmul(tq[i * ld + r], Bp[i * nb + j]));
-    const int64_t o = (p * ng + gt + r) * nb + j;
-    dwr[o] = acc.x;
-    dwi[o] = acc.y;
+  const double2* Qb = qbar + p * nb * ng;
+  const double2* Q = q + p * nb * ng;
+  double* Dr = dwr + p * ng * nb;
+  double* Di = dwi + p * ng * nb;
+  const int gb = blockIdx.x * kApplyRows;
+  const int ge = min(ng, gb + kApplyRows);
+  const int nm = (nb + 3) / 4;
+  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
+    d4 re[T], im[T];
+#pragma unroll
+    for (int J = 0; J < T; ++J) { re[J] = d4zero(); im[J] = d4zero(); }
+    const int g = g0 + li;
+    for (int m = 0; m < nm; ++m) {
+      const int band = 4 * m + lk;
+      const bool ok = g < ge && band < nb;
+      const double2 vb = ok ? Qb[(int64_t)band * ng + g] : czero();
+      const double2 vq = ok ? Q[(int64_t)band * ng + g] : czero();
+#pragma unroll
+      for (int J = 0; J < T; ++J) {
+        const double2 a = sa[(4 * m + lk) * NBP + 16 * J + li];
+        const double2 b = sb[(4 * m + lk) * NBP + 16 * J + li];
+        re[J] = mfma4(vb.x, a.x, re[J]);
+        re[J] = mfma4(-vb.y, a.y, re[J]);
+        re[J] = mfma4(vq.x, b.x, re[J]);
+        re[J] = mfma4(-vq.y, b.y, re[J]);
+        im[J] = mfma4(vb.x, a.y, im[J]);
+        im[J] = mfma4(vb.y, a.x, im[J]);
+        im[J] = mfma4(vq.x, b.y, im[J]);
+        im[J] = mfma4(vq.y, b.x, im[J]);
+      }
+    }
+#pragma unroll
+    for (int J = 0; J < T; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = g0 + 4 * r + lk, col = 16 * J + li;
+        if (row < ge && col < nb) {
+          Dr[(int64_t)row * nb + col] = re[J][r];
+          Di[(int64_t)row * nb + col] = im[J][r];
+        }
+      }
   }
 }
 
 int launch_apply_bwd(const double2* qbar, const double2* q, const double2* A,
                      const double2* Bm, double* dwr, double* dwi, int64_t P,
                      int ng, int nb, hipStream_t s) {
-  size_t shm = 2 * (size_t)nb * (kQrRows + 1) * sizeof(double2);
-  dim3 grid(ceil_div(ng, kQrRows), (unsigned)P), block(kQrBlock);
-  hipLaunchKernelGGL(apply_bwd_kernel, grid, block, shm, s, qbar, q, A, Bm,
-                     dwr, dwi, ng, nb);
+  const int T = (nb + 15) / 16;
+  size_t shm = 2 * (size_t)(16 * T) * (16 * T) * sizeof(double2);
+  dim3 grid(ceil_div(ng, kApplyRows), (unsigned)P), block(kQrBlock);
+  switch (T) {
+    case 1: hipLaunchKernelGGL((apply_bwd_kernel<1>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+    case 2: hipLaunchKernelGGL((apply_bwd_kernel<2>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+    case 3: hipLaunchKernelGGL((apply_bwd_kernel<3>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+    case 4: hipLaunchKernelGGL((apply_bwd_kernel<4>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+    default:
+      set_error("nb above 64 is not supported by this build");
+      return JRX_ERR_UNSUPPORTED;
+  }
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

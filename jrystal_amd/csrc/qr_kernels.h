@@ -20,13 +20,15 @@ struct GramSplit {
 };
 GramSplit gram_split(int64_t P, int ng);
 
-// partial[p][c] = sum over chunk c of conj(a_g)^T a_g, a_g = W[p][g][:] (times
-// pre[p] on the right when pre != nullptr).
+// partial[p][4c + w] = sum over the rows of chunk c taken by wave w of
+// conj(a_g)^T a_g, a_g = W[p][g][:] (times pre[p] on the right when
+// pre != nullptr).  Four partials per chunk.
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
                        const GramSplit& sp, hipStream_t s);
 
-// partial[p][c] = sum over chunk c of conj(Q[p][:][g]) (x) Qb[p][:][g].
+// partial[p][4c + w] = sum over wave w's rows of chunk c of
+// conj(Q[p][:][g]) (x) Qb[p][:][g].
 int launch_gram_bmajor(const double2* q, const double2* qb, double2* partial,
                        int64_t P, int ng, int nb, const GramSplit& sp,
                        hipStream_t s);

@@ -67,6 +67,16 @@ def test_si8_cube_64_k8():
   _case(W.si8_cube(14.0, k=2))
 
 
+@pytest.mark.parametrize("nb", [17, 48, 64])
+def test_band_tiles(nb):
+  """Band counts that fill 2, 3 and 4 MFMA tiles, with padding."""
+  from jrystal_amd import workloads as W
+  wl = W.build(f"si2-16^3-{nb}b", W.fcc_si2(14.0), (16, 16, 16), (1, 1, 2),
+               nb, 11.0)
+  assert wl.num_g > nb
+  _case(wl, seed=nb, e_tol=1e-9, rho_tol=1e-11, g_tol=1e-9)
+
+
 def test_chunked_matches_unchunked():
   from jrystal_amd import workloads as W
   wl = W.si2_fcc(14.0)
