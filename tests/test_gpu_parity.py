@@ -141,3 +141,25 @@ def test_autograd_function():
   ref = plan.evaluate(wr.detach(), wi.detach(), grad=True)
   assert torch.allclose(wr.grad, 2.5 * ref["dw_re"], rtol=0, atol=1e-13)
   assert torch.allclose(wi.grad, 2.5 * ref["dw_im"], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["tiny_z4", "si2_z14", "si2_z4"])
+def test_golden_vectors(name):
+  """GPU path against the committed oracle vectors (tests/golden)."""
+  import os
+  from jrystal_amd import workloads as W
+  gold = np.load(os.path.join(os.path.dirname(__file__), "golden",
+                              "oracle_golden.npz"))
+  wl = {"tiny_z4": lambda: W.tiny(z=4.0), "si2_z14": lambda: W.si2_fcc(14.0),
+        "si2_z4": lambda: W.si2_fcc(4.0)}[name]()
+  out = _run(_plan(wl.crystal, wl.mask, wl.kpts, wl.occupation),
+             wl.params(0))
+  assert np.max(np.abs(out["energies"] - gold[f"{name}/energies"])) < E_TOL
+  rho = out["density"]
+  np.testing.assert_allclose([rho.sum(), (rho**2).sum(), rho.max()],
+                             gold[f"{name}/rho_sum"], rtol=1e-12)
+  t = tuple(gold[f"{name}/grad_idx"].T)
+  norm = gold[f"{name}/grad_norm"]
+  for part, n in (("re", norm[0]), ("im", norm[1])):
+    got = out[f"dw_{part}"][t]
+    assert np.max(np.abs(got - gold[f"{name}/grad_{part}"])) < G_TOL * n
