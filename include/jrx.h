@@ -96,6 +96,86 @@ int jrx_energy_grad(jrx_plan* plan, const double* w_re, const double* w_im,
                     size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Stand-alone operators behind the reference's public functions (full-box
+ * layout).  Complex arrays interleaved; "state" = (spin, kpt, band) flattened
+ * in that order; N = nx*ny*nz.  Gradients follow the convention
+ * dL/dRe z + i dL/dIm z and are scaled by grad_out (device scalar, or
+ * per-state array where noted).
+ * ---------------------------------------------------------------------- */
+/* unitary_module.unitary_matrix (unitary_module.py:30-81) for P problems:
+ * q [P][nb][ng] (band-major) orthonormal columns of W = w_re + i w_im
+ * ([P][ng][nb]); rinv [P][nb][nb] = R^-1 (upper), kept for the backward. */
+size_t jrx_qr_workspace_bytes(int64_t P, int ng, int nb);
+int jrx_qr(const double* w_re, const double* w_im, int64_t P, int ng, int nb,
+           double* q, double* rinv, void* workspace, size_t workspace_bytes,
+           void* stream);
+int jrx_qr_backward(const double* q, const double* qbar, const double* rinv,
+                    int64_t P, int ng, int nb, double* dw_re, double* dw_im,
+                    void* workspace, size_t workspace_bytes, void* stream);
+/* utils.expand_coefficient / squeeze_coefficient (utils.py:257-308):
+ * q [states][ng] <-> box [states][N]; mask_index [ng] = flat C-order box
+ * offsets of the mask points. */
+int jrx_expand_coefficient(const double* q, const int64_t* mask_index,
+                           double* box, int64_t nstates, int ng, int64_t N,
+                           void* stream);
+int jrx_squeeze_coefficient(const double* box, const int64_t* mask_index,
+                            double* q, int64_t nstates, int ng, int64_t N,
+                            void* stream);
+/* pw.density_grid after wave_grid (pw.py:273-284): out [ns][N] =
+ * sum_{k,b} occ |psi|^2, or out [states][N] = |psi|^2 when occ is NULL. */
+int jrx_density(const double* psi, const double* occ, double* out, int ns,
+                int64_t states_per_spin, int64_t N, void* stream);
+int jrx_density_backward(const double* psi, const double* occ,
+                         const double* grad_rho, double* grad_psi, int ns,
+                         int64_t states_per_spin, int64_t N, void* stream);
+/* energy.kinetic (energy.py:138-182) on the box: gvec [N][3], kpts [nk][3];
+ * out[0] = 1/2 sum occ sum_G |G+k|^2 |C|^2 (occ given) or out[state] =
+ * 1/2 sum_G |G+k|^2 |C|^2.  grad_out: scalar (occ given) or per state. */
+size_t jrx_kinetic_workspace_bytes(int64_t nstates, int64_t N);
+int jrx_kinetic_box(const double* coeff, const double* gvec,
+                    const double* kpts, const double* occ, int ns, int nk,
+                    int nb, int64_t N, double* out, void* workspace,
+                    size_t workspace_bytes, void* stream);
+int jrx_kinetic_box_backward(const double* coeff, const double* gvec,
+                             const double* kpts, const double* occ,
+                             const double* grad_out, int ns, int nk, int nb,
+                             int64_t N, double* grad_coeff, void* stream);
+/* energy.hartree / external / xc_energy (energy.py:29-135,185-211) of a
+ * density: rho_rec [ns][N] complex, rho [ns][N] real.  grad (may be NULL)
+ * receives dE/drho_rec (complex) or dE/drho (real) times grad_out[0]. */
+size_t jrx_reduce_workspace_bytes(int64_t n);
+int jrx_hartree(const double* rho_rec, const double* gvec, int ns, int64_t N,
+                double vol, int kohn_sham, double* out, double* grad,
+                const double* grad_out, void* workspace,
+                size_t workspace_bytes, void* stream);
+int jrx_external(const double* rho_rec, const double* gvec,
+                 const double* positions, const double* charges, int natom,
+                 int ns, int64_t N, double* out, double* grad,
+                 const double* grad_out, void* workspace,
+                 size_t workspace_bytes, void* stream);
+int jrx_xc_lda(const double* rho, int ns, int64_t N, double vol, double* out,
+               double* grad, const double* grad_out, void* workspace,
+               size_t workspace_bytes, void* stream);
+/* Element-wise helpers of the autograd glue: y = (x, 0); y = Re x (n
+ * complex); y = alpha[0] * x (alpha a device scalar). */
+int jrx_real_to_complex(const double* x, double* y, int64_t n, void* stream);
+int jrx_complex_real(const double* x, double* y, int64_t n, void* stream);
+int jrx_scale(const double* x, const double* alpha, double* y, int64_t n,
+              void* stream);
+/* One optax.adam step (opt_utils.py:153-168 -> optax.adam) in place on a
+ * flat float64 parameter; m, v are the moment buffers, step is 1-based. */
+int jrx_adam_update(double* param, const double* grad, double* m, double* v,
+                    int64_t n, double lr, double b1, double b2, double eps,
+                    double eps_root, int64_t step, void* stream);
+/* ewald.ewald_coulomb_repulsion (ewald.py:21-86) with the caller's
+ * translation list [nt][3] and G grid [N][3]. */
+size_t jrx_ewald_workspace_bytes(int64_t n_translations, int64_t N);
+int jrx_ewald(const double* positions, const double* charges, int natom,
+              const double* gvec, int64_t N, double vol, double eta,
+              const double* translations, int64_t nt, double* out,
+              void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Per-kernel timing of jrx_energy_grad with HIP events recorded on the
  * call's stream (no counterpart in the reference, which has no profiling;
  * used by bench.py for the roofline of the dominant kernel).

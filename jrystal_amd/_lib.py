@@ -54,6 +54,26 @@ SIGNATURES = {
   "jrx_plan_workspace_bytes": [_P, ctypes.POINTER(_SZ)],
   "jrx_plan_get_info": [_P, ctypes.POINTER(JrxPlanInfo)],
   "jrx_energy_grad": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_qr_workspace_bytes": [_I64, _I, _I],
+  "jrx_qr": [_P, _P, _I64, _I, _I, _P, _P, _P, _SZ, _P],
+  "jrx_qr_backward": [_P, _P, _P, _I64, _I, _I, _P, _P, _P, _SZ, _P],
+  "jrx_expand_coefficient": [_P, _P, _P, _I64, _I, _I64, _P],
+  "jrx_squeeze_coefficient": [_P, _P, _P, _I64, _I, _I64, _P],
+  "jrx_density": [_P, _P, _P, _I, _I64, _I64, _P],
+  "jrx_density_backward": [_P, _P, _P, _P, _I, _I64, _I64, _P],
+  "jrx_kinetic_workspace_bytes": [_I64, _I64],
+  "jrx_kinetic_box": [_P, _P, _P, _P, _I, _I, _I, _I64, _P, _P, _SZ, _P],
+  "jrx_kinetic_box_backward": [_P, _P, _P, _P, _P, _I, _I, _I, _I64, _P, _P],
+  "jrx_reduce_workspace_bytes": [_I64],
+  "jrx_hartree": [_P, _P, _I, _I64, _D, _I, _P, _P, _P, _P, _SZ, _P],
+  "jrx_external": [_P, _P, _P, _P, _I, _I, _I64, _P, _P, _P, _P, _SZ, _P],
+  "jrx_xc_lda": [_P, _I, _I64, _D, _P, _P, _P, _P, _SZ, _P],
+  "jrx_adam_update": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _D, _I64, _P],
+  "jrx_real_to_complex": [_P, _P, _I64, _P],
+  "jrx_complex_real": [_P, _P, _I64, _P],
+  "jrx_scale": [_P, _P, _P, _I64, _P],
+  "jrx_ewald_workspace_bytes": [_I64, _I64],
+  "jrx_ewald": [_P, _P, _I, _P, _I64, _D, _D, _P, _I64, _P, _P, _SZ, _P],
   "jrx_plan_set_timing": [_P, _I],
   "jrx_plan_read_timing": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
 }
@@ -65,7 +85,11 @@ KERNEL_CLASSES = (
   "qr_bwd_apply", "other",
 )
 _RESTYPES = {"jrx_last_error": ctypes.c_char_p,
-             "jrx_version": ctypes.c_char_p}
+             "jrx_version": ctypes.c_char_p,
+             "jrx_qr_workspace_bytes": ctypes.c_size_t,
+             "jrx_kinetic_workspace_bytes": ctypes.c_size_t,
+             "jrx_reduce_workspace_bytes": ctypes.c_size_t,
+             "jrx_ewald_workspace_bytes": ctypes.c_size_t}
 
 
 def lib():

@@ -1,0 +1,157 @@
+"""All-electron ground-state energy driver on the GPU.
+
+Mirrors jrystal/calc/calc_ground_state_energy_all_electrons.py:63-247:
+build crystal / grids / mask / k-points / Ewald constant / occupation from
+the config, initialise the plane-wave parameters, then run the optimiser on
+the free energy F = E - T S with the temperature annealing schedule and the
+convergence window.  Each step's energy and gradient come from one fused
+evaluation (jrystal_amd.objective.EnergyPlan); the update is the fused Adam
+kernel.  Returns the density, as the reference does.
+"""
+import logging
+import math
+import os
+import time
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+import torch
+
+from .. import energy, entropy, grid, occupation, pw
+from ..config import JrystalConfigDict
+from ..crystal import Crystal
+from ..objective import ENERGY_KEYS, EnergyPlan
+from ..optim import Adam, ConvergenceChecker, exponential_decay
+
+log = logging.getLogger("jrystal_amd")
+
+GEOMETRY_DIR = os.path.join(os.path.dirname(os.path.dirname(
+  os.path.abspath(__file__))), "geometry")
+
+
+@dataclass
+class GroundStateEnergyOutput:
+  """calc_ground_state_energy_all_electrons.py:43-60, plus the breakdown."""
+  config: JrystalConfigDict
+  crystal: Crystal
+  params_pw: dict
+  params_occ: np.ndarray
+  total_energy: float
+  total_energy_history: List[float] = field(default_factory=list)
+  energies: dict = field(default_factory=dict)
+  density: object = None
+  converged: bool = False
+  steps: int = 0
+  seconds: float = 0.0
+
+
+def create_crystal(config) -> Crystal:
+  """opt_utils.py:106-114."""
+  if config.crystal is not None:
+    path = os.path.join(GEOMETRY_DIR, config.crystal + ".xyz")
+  else:
+    path = config.crystal_file_path_path
+  cr = Crystal.create_from_file(path, spin=config.spin)
+  if cr.num_electron % 2 != cr.spin % 2:  # utils.check_spin_number
+    raise ValueError("spin number is not valid for the system. ")
+  return cr
+
+
+def create_freq_mask(config, crystal):
+  """opt_utils.py:76-103."""
+  gs = grid.proper_grid_size(config.grid_sizes)
+  if config.freq_mask_method == "cubic":
+    return grid.cubic_mask(gs)
+  if config.freq_mask_method == "spherical":
+    return grid.spherical_mask(crystal.cell_vectors, gs, config.cutoff_energy)
+  raise ValueError("freq_mask_method must be either cubic or spherical.")
+
+
+def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
+  if config.use_pseudopotential:
+    raise NotImplementedError("pseudopotential energies are outside the "
+                              "all-electron path")
+  if config.occupation not in ("uniform", "gamma"):
+    raise NotImplementedError(
+      f"occupation {config.occupation!r}: only the fixed uniform / gamma "
+      "occupations are on this path")
+  if config.optimizer != "adam":
+    raise NotImplementedError(f"optimizer {config.optimizer!r}")
+  if config.scheduler:
+    raise NotImplementedError("Scheduler is not implemented yet.")
+  cr = create_crystal(config)
+  ne = cr.num_electron
+  gs = grid.proper_grid_size(config.grid_sizes)
+  ks = grid.proper_grid_size(config.k_grid_sizes)
+  g_vec = grid.g_vectors(cr.cell_vectors, gs)
+  kpts = grid.k_vectors(cr.cell_vectors, ks)
+  nk = kpts.shape[0]
+  num_bands = math.ceil(ne / 2) + config.empty_bands
+  mask = create_freq_mask(config, cr)
+  ew = float(energy.nuclear_repulsion(
+    cr.positions, cr.charges, cr.cell_vectors, g_vec, cr.vol,
+    config.ewald_args["ewald_eta"], config.ewald_args["ewald_cutoff"]))
+  occ = occupation.param_init(None, num_bands, ne, nk, cr.spin,
+                              config.occupation, config.spin_restricted)
+  etro = entropy.fermi_dirac(occ, eps=config.eps)
+  plan = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, mask, kpts,
+                    occ, xc=config.xc, chunk_states=config.chunk_states)
+  params = pw.param_init(config.seed, num_bands, nk, mask,
+                         spin_restricted=config.spin_restricted,
+                         dist=config.param_dist)
+  opt_args = dict(config.optimizer_args)
+  opt = Adam(params, **opt_args)
+  if config.smearing > 0.0:
+    temperature = exponential_decay(100.0, config.epoch // 2,
+                                    config.smearing / 100, config.smearing)
+  else:
+    def temperature(i):
+      return 0.0
+  checker = ConvergenceChecker(config.convergence_window_size,
+                               config.convergence_condition)
+  log.info("crystal %s: %d electrons, %d bands, %d k-points, G_n=%d",
+           cr.symbols, ne, num_bands, nk, plan.num_g)
+  history = []
+  converged = False
+  out = {}
+  t0 = time.perf_counter()
+  i = 0
+  for i in range(config.epoch):
+    temp = temperature(i)
+    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+    opt.step(params, {"w_re": out["dw_re"], "w_im": out["dw_im"]})
+    etot = float(out["energies"][4])  # one host sync per step
+    history.append(etot)
+    if not np.isfinite(etot):
+      raise FloatingPointError(f"energy became {etot} at step {i}")
+    if checker.check(etot):
+      converged = True
+      log.info("Converged.")
+      break
+    if config.verbose and i % 100 == 0:
+      log.info("step %d | Loss %.6f | Energy %.6f | Entropy %.4f | T %.2E",
+               i, etot - temp * etro, etot + ew, etro, temp)
+  seconds = time.perf_counter() - t0
+  if not converged:
+    log.warning("Did not converge.")
+  final = plan.evaluate(params["w_re"], params["w_im"], grad=False,
+                        density=True)
+  e = dict(zip(ENERGY_KEYS, final["energies"].tolist()))
+  e["nuclear_repulsion"] = ew
+  e["total_with_nuclear"] = e["total"] + ew
+  for name, key in (("Hartree", "hartree"), ("External", "external"),
+                    ("XC", "xc"), ("Kinetic", "kinetic"),
+                    ("Nuclear repulsion", "nuclear_repulsion")):
+    log.info("%s Energy: %.4f Ha", name, e[key])
+  log.info("Total Energy: %.4f Ha", e["total_with_nuclear"])
+  return GroundStateEnergyOutput(
+    config=config, crystal=cr, params_pw=params, params_occ=occ,
+    total_energy=e["total_with_nuclear"], total_energy_history=history,
+    energies=e, density=final["density"], converged=converged, steps=i + 1,
+    seconds=seconds)
+
+
+def calc(config: JrystalConfigDict):
+  """Returns the density (spin, x, y, z), as the reference driver does."""
+  return calc_with_output(config).density

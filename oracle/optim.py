@@ -1,0 +1,52 @@
+"""Optimiser restatements (oracle side): optax.adam, exponential_decay and
+the convergence window of the reference driver.
+
+optax is a reference dependency (requirements.txt:3) absent here; its
+published update rule is restated:
+  m = b1 m + (1-b1) g;  v = b2 v + (1-b2) g^2;  t += 1
+  p -= lr * (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t) + eps_root) + eps)
+Step-level bitwise agreement with optax is parity unpinned (rounding only).
+"""
+import numpy as np
+
+
+class Adam:
+
+  def __init__(self, params, learning_rate=0.01, b1=0.9, b2=0.999, eps=1e-8,
+               eps_root=0.0):
+    self.lr, self.b1, self.b2, self.eps, self.eps_root = (
+      learning_rate, b1, b2, eps, eps_root)
+    self.m = {k: np.zeros_like(v) for k, v in params.items()}
+    self.v = {k: np.zeros_like(v) for k, v in params.items()}
+    self.t = 0
+
+  def step(self, params, grads):
+    self.t += 1
+    c1 = 1 - self.b1**self.t
+    c2 = 1 - self.b2**self.t
+    out = {}
+    for k, p in params.items():
+      g = grads[k]
+      self.m[k] = self.b1 * self.m[k] + (1 - self.b1) * g
+      self.v[k] = self.b2 * self.v[k] + (1 - self.b2) * g * g
+      out[k] = p - self.lr * (self.m[k] / c1) / (
+        np.sqrt(self.v[k] / c2 + self.eps_root) + self.eps)
+    return out
+
+
+def exponential_decay(init_value, transition_steps, decay_rate,
+                      end_value=None):
+  """optax.exponential_decay, transition_begin = 0, no staircase."""
+  def f(count):
+    v = init_value * decay_rate**(count / transition_steps)
+    if end_value is not None:
+      v = max(v, end_value) if decay_rate < 1 else min(v, end_value)
+    return v
+  return f
+
+
+def converged(history, window, threshold):
+  """calc/convergence.py:20-32 applied to a full history."""
+  if len(history) < window:
+    return False
+  return bool(np.std(history[-window:]) < threshold)

@@ -1,0 +1,81 @@
+"""Host-side logic of the drop-in layer: config, schedules, convergence,
+entropy, CLI dispatch (CPU only)."""
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import occupation as OO
+from oracle import optim as OOpt
+
+
+def test_config_defaults_and_merge(tmp_path):
+  from jrystal_amd import config
+  cfg = config.get_config()
+  assert cfg.xc == "lda_x" and cfg.ewald_args["ewald_eta"] == 0.1
+  p = tmp_path / "c.yaml"
+  p.write_text("crystal: si\ncutoff_energy: 30\newald_args:\n  ewald_eta: 0.2\n"
+               "unknown_key: 1\n")
+  with warnings.catch_warnings(record=True) as w:
+    warnings.simplefilter("always")
+    cfg = config.get_config(str(p))
+  assert any("unknown_key" in str(x.message) for x in w)
+  assert cfg.crystal == "si" and cfg.cutoff_energy == 30
+  assert cfg.ewald_args == {"ewald_eta": 0.2, "ewald_cutoff": 2e4}
+  assert cfg.epoch == 5000  # default merged under the file's keys
+  assert cfg.band_structure_empty_bands == cfg.empty_bands or \
+      cfg.band_structure_empty_bands == 8
+
+
+def test_exponential_decay_matches_oracle():
+  from jrystal_amd.optim import exponential_decay
+  a = exponential_decay(100.0, 2500, 0.001 / 100, 0.001)
+  b = OOpt.exponential_decay(100.0, 2500, 0.001 / 100, 0.001)
+  for i in (0, 1, 100, 2500, 4999, 10000):
+    assert a(i) == b(i)
+  assert a(0) == 100.0 and a(10**6) == 0.001
+  np.testing.assert_allclose(a(2500), 100.0 * 1e-5)
+
+
+def test_convergence_checker():
+  from jrystal_amd.optim import ConvergenceChecker
+  c = ConvergenceChecker(3, 1e-3)
+  assert not c.check(1.0) and not c.check(1.0)
+  assert c.check(1.0)
+  c = ConvergenceChecker(3, 1e-3)
+  vals = [5.0, 4.0, 3.0, 2.9999, 2.99995, 2.99996]
+  got = [c.check(v) for v in vals]
+  want = [OOpt.converged(vals[:i + 1], 3, 1e-3) for i in range(len(vals))]
+  assert got == want
+
+
+def test_entropy_matches_oracle():
+  from jrystal_amd import entropy
+  for occ in (OO.uniform(8, 28, 0, 8), OO.uniform(4, 6, 0, 6),
+              OO.uniform(2, 5, 1, 4, spin_restricted=False)):
+    assert entropy.fermi_dirac(occ) == OO.fermi_dirac_entropy(occ)
+
+
+def test_cli_band_mode_is_outside_the_path(tmp_path):
+  from jrystal_amd.__main__ import main
+  p = tmp_path / "c.yaml"
+  p.write_text("verbose: false\n")
+  with pytest.raises(NotImplementedError):
+    main(["-m", "band", "-c", str(p)])
+
+
+def test_driver_rejects_unsupported_options():
+  pytest.importorskip("torch")
+  from jrystal_amd import config
+  from jrystal_amd.calc.energy_all_electrons import calc_with_output
+  cfg = config.get_config()
+  cfg.occupation = "simplex-projector"
+  with pytest.raises(NotImplementedError):
+    calc_with_output(cfg)
+
+
+def test_geometry_files_shipped():
+  from jrystal_amd.calc.energy_all_electrons import GEOMETRY_DIR
+  for n in ("si", "si8", "diamond"):
+    assert os.path.exists(os.path.join(GEOMETRY_DIR, n + ".xyz"))
