@@ -35,8 +35,8 @@ def algorithmic_bytes(cls, plan, chunks):
   Q = S * ng * 16                 # orthonormal coefficients
   W = 2 * S * ng * 8              # planar parameters (or gradient)
   rho = plan.ns * N * 8
-  if cls == "plane_ifft":         # two sweeps, each reads Q, writes T
-    return 2 * (Q + T)
+  if cls == "plane_ifft":         # reads Q, writes T, once per sweep
+    return (1 if plan.keep_planes else 2) * (Q + T)
   if cls == "xline_density":      # read T; rho read+write once per chunk
     return T + chunks * 2 * rho
   if cls == "xline_vpsi":         # read T, write T, read v once per chunk
@@ -156,7 +156,7 @@ def main():
     plan.evaluate(wr, wi, grad=True, out=out)
   timing = plan.read_timing()
   plan.set_timing(False)
-  chunks = -(-plan.ns * plan.nk * plan.nb // plan.chunk_states)
+  chunks = plan.ns * max(1, -(-plan.nk * plan.nb // plan.chunk_states))
   dom = max(timing, key=lambda k: timing[k][0])
   ms, launches = timing[dom]
   per_eval_ms = ms / nprof
@@ -189,6 +189,7 @@ def main():
                "num_kpts": int(plan.nk), "num_bands": int(plan.nb),
                "num_g": int(plan.num_g), "cutoff_Ha": wl.cutoff_energy,
                "chunk_states": int(plan.chunk_states),
+               "keep_planes": bool(plan.keep_planes),
                "energy_total_Ha": energies[4]},
     "roofline": roofline,
     "kernel_ms_per_eval": breakdown,

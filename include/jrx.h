@@ -79,6 +79,7 @@ typedef struct {
   int ng;             /* number of mask points (G_n)                        */
   int active_planes;  /* x-planes of the box that hold mask points          */
   int chunk_states;   /* states per FFT chunk                               */
+  int keep_planes;    /* 1: the adjoint sweep reuses the forward planes     */
   size_t workspace_bytes;
 } jrx_plan_info;
 int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info);

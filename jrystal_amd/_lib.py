@@ -32,7 +32,7 @@ class JrxProblem(ctypes.Structure):
 
 class JrxPlanInfo(ctypes.Structure):
   _fields_ = [("ng", ctypes.c_int), ("active_planes", ctypes.c_int),
-              ("chunk_states", ctypes.c_int),
+              ("chunk_states", ctypes.c_int), ("keep_planes", ctypes.c_int),
               ("workspace_bytes", ctypes.c_size_t)]
 
 

@@ -436,4 +436,273 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
   return JRX_OK;
 }
 
+
+// ----------------------------------------- register kernels for 64-point --
+// Specialisations for nx = ny = nz = 64 built on fft64_8x8: no zero fills,
+// no block barriers inside the transforms, loads and stores straight from
+// registers.  Plane rows are padded to 66 entries so that the column
+// accesses of 16-lane groups hit distinct LDS banks.
+constexpr int kPL = 66;
+constexpr int kExLine = 72;  // padded exchange slots per x-line
+
+__global__ __launch_bounds__(kPlaneBlock) void plane_ifft64_kernel(
+    const double2* __restrict__ q, double2* __restrict__ T, int64_t state0,
+    BoxGeom geo, const double2* __restrict__ tw64) {
+  extern __shared__ double2 sm[];
+  double2* pl = sm;              // [64][kPL]
+  double2* stw = sm + 64 * kPL;  // [64]
+  const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
+  const int xi = blockIdx.x, c = blockIdx.y;
+  const int64_t st = state0 + c;
+  for (int i = tid; i < 64 * kPL; i += kPlaneBlock) pl[i] = czero();
+  if (tid < 64) stw[tid] = tw64[tid];
+  __syncthreads();
+  const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
+  const double2* qs = q + st * geo.ng;
+  for (int g = g0 + tid; g < g1; g += kPlaneBlock) {
+    const int yz = geo.g_yz[g];
+    pl[(yz >> 6) * kPL + (yz & 63)] = qs[g];
+  }
+  __syncthreads();
+  const int r0 = geo.row_off[xi], nr = geo.row_off[xi + 1] - r0;
+  if (line < nr) {  // z transform of the mask rows
+    double2* base = pl + geo.rows[r0 + line] * kPL;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = base[j + 8 * m];
+    fft64_8x8<true>(v, base, 1, j, stw);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) base[j + 8 * t] = v[t];
+  }
+  __syncthreads();
+  {  // y transform of column z = line, straight to the plane buffer T
+    double2* base = pl + line;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = base[(j + 8 * m) * kPL];
+    fft64_8x8<true>(v, base, kPL, j, stw);
+    double2* dst = T + ((int64_t)c * geo.nxa + xi) * 4096 + line;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) dst[(j + 8 * t) * 64] = v[t];
+  }
+}
+
+__global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather64_kernel(
+    const double2* __restrict__ T, const double2* __restrict__ q,
+    double2* __restrict__ qbar, double* __restrict__ kin_partial,
+    int64_t state0, BoxGeom geo, const double2* __restrict__ tw64,
+    GatherParams gp) {
+  extern __shared__ double2 sm[];
+  double2* pl = sm;
+  double2* stw = sm + 64 * kPL;
+  const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
+  const int xi = blockIdx.x, c = blockIdx.y;
+  const int64_t st = state0 + c;
+  if (tid < 64) stw[tid] = tw64[tid];
+  __syncthreads();
+  {  // y transform of column z = line, loaded straight from T
+    const double2* src = T + ((int64_t)c * geo.nxa + xi) * 4096 + line;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = src[(j + 8 * m) * 64];
+    double2* base = pl + line;
+    fft64_8x8<false>(v, base, kPL, j, stw);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) base[(j + 8 * t) * kPL] = v[t];
+  }
+  __syncthreads();
+  const int r0 = geo.row_off[xi], nr = geo.row_off[xi + 1] - r0;
+  if (line < nr) {  // z transform of the mask rows
+    double2* base = pl + geo.rows[r0 + line] * kPL;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = base[j + 8 * m];
+    fft64_8x8<false>(v, base, 1, j, stw);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) base[j + 8 * t] = v[t];
+  }
+  __syncthreads();
+  const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
+  const int k = (int)((st / gp.nb) % gp.nk);
+  const double f = gp.occ[st];
+  const int mx = fftfreq_index(geo.plane_x[xi], 64);
+  const double ax = mx * gp.B[0] + gp.kpts[3 * k];
+  const double ay = mx * gp.B[1] + gp.kpts[3 * k + 1];
+  const double az = mx * gp.B[2] + gp.kpts[3 * k + 2];
+  const double2* qs = q + st * geo.ng;
+  double2* qb = qbar + st * geo.ng;
+  double kin = 0.0;
+  for (int g = g0 + tid; g < g1; g += kPlaneBlock) {
+    const int yz = geo.g_yz[g];
+    const int iy = yz >> 6, iz = yz & 63;
+    const int my = fftfreq_index(iy, 64), mz = fftfreq_index(iz, 64);
+    const double gx = ax + my * gp.B[3] + mz * gp.B[6];
+    const double gy = ay + my * gp.B[4] + mz * gp.B[7];
+    const double gz = az + my * gp.B[5] + mz * gp.B[8];
+    const double gk2 = gx * gx + gy * gy + gz * gz;
+    const double2 cv = qs[g];
+    const double2 hv = pl[iy * kPL + iz];
+    qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
+                         f * (gp.two_over_vol * hv.y + gk2 * cv.y));
+    kin += f * gk2 * (cv.x * cv.x + cv.y * cv.y);
+  }
+  __syncthreads();
+  const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
+  if (tid == 0) kin_partial[st * geo.nxa + xi] = tot;
+}
+
+// x-lines: block = 32 lines (fixed y, z0..z0+31), 8 lanes per line.
+__global__ __launch_bounds__(kXBlock) void xline_density64_kernel(
+    const double2* __restrict__ T, double* __restrict__ rho_part,
+    const double* __restrict__ occ, int64_t state0, int nstates,
+    double inv_vol, BoxGeom geo, const double2* __restrict__ tw64) {
+  __shared__ double2 ex[(kXBlock / 8) * kExLine];
+  __shared__ double2 stw[64];
+  const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
+  const int nz = geo.nz, ny = geo.ny, P = ny * nz;
+  const int nzt = (nz + 31) / 32;
+  const int y = blockIdx.x / nzt;
+  const int z = (blockIdx.x % nzt) * 32 + line;
+  const bool zok = z < nz;
+  const int G = gridDim.y, gi = blockIdx.y;
+  const int per = (nstates + G - 1) / G;
+  const int cb = gi * per, ce = min(nstates, cb + per);
+  if (tid < 64) stw[tid] = tw64[tid];
+  __syncthreads();
+  if (cb >= ce) return;
+  int xo[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int xi = geo.xmap[j + 8 * m];
+    xo[m] = (xi >= 0 && zok) ? xi * P : -1;
+  }
+  double2* base = ex + line * kExLine;
+  double acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) acc[t] = 0.0;
+  const double2* src0 = T + (int64_t)y * nz + z;
+  double2 nxt[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    nxt[m] = xo[m] >= 0 ? src0[(int64_t)cb * geo.nxa * P + xo[m]] : czero();
+  for (int c = cb; c < ce; ++c) {
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = nxt[m];
+    if (c + 1 < ce) {
+      const double2* src = src0 + (int64_t)(c + 1) * geo.nxa * P;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) nxt[m] = xo[m] >= 0 ? src[xo[m]] : czero();
+    }
+    fft64_8x8<true>(v, base, 1, j, stw);
+    const double f = occ[state0 + c];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] += f * (v[t].x * v[t].x + v[t].y * v[t].y);
+  }
+  if (!zok) return;
+  double* part = rho_part + (int64_t)gi * 64 * P + (int64_t)y * nz + z;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) part[(int64_t)(j + 8 * t) * P] += acc[t] * inv_vol;
+}
+
+__global__ __launch_bounds__(kXBlock) void xline_vpsi64_kernel(
+    double2* __restrict__ T, const double* __restrict__ v, int nstates,
+    BoxGeom geo, const double2* __restrict__ tw64) {
+  __shared__ double2 ex[(kXBlock / 8) * kExLine];
+  __shared__ double2 stw[64];
+  const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
+  const int nz = geo.nz, ny = geo.ny, P = ny * nz;
+  const int nzt = (nz + 31) / 32;
+  const int y = blockIdx.x / nzt;
+  const int z = (blockIdx.x % nzt) * 32 + line;
+  const bool zok = z < nz;
+  const int G = gridDim.y, gi = blockIdx.y;
+  const int per = (nstates + G - 1) / G;
+  const int cb = gi * per, ce = min(nstates, cb + per);
+  if (tid < 64) stw[tid] = tw64[tid];
+  __syncthreads();
+  if (cb >= ce) return;
+  int xo[8];
+  double vr[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int xi = geo.xmap[j + 8 * m];
+    xo[m] = (xi >= 0 && zok) ? xi * P : -1;
+    vr[m] = zok ? v[(int64_t)(j + 8 * m) * P + (int64_t)y * nz + z] : 0.0;
+  }
+  double2* base = ex + line * kExLine;
+  double2* buf0 = T + (int64_t)y * nz + z;
+  double2 nxt[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    nxt[m] = xo[m] >= 0 ? buf0[(int64_t)cb * geo.nxa * P + xo[m]] : czero();
+  for (int c = cb; c < ce; ++c) {
+    double2 w[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) w[m] = nxt[m];
+    double2* buf = buf0 + (int64_t)c * geo.nxa * P;
+    if (c + 1 < ce) {
+      const double2* src = buf + (int64_t)geo.nxa * P;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) nxt[m] = xo[m] >= 0 ? src[xo[m]] : czero();
+    }
+    fft64_8x8<true>(w, base, 1, j, stw);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) w[t] = cscale(w[t], vr[t]);
+    fft64_8x8<false>(w, base, 1, j, stw);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      if (xo[m] >= 0) buf[xo[m]] = w[m];
+  }
+}
+
+bool use_reg64(const BoxGeom& geo) {
+  return geo.nx == 64 && geo.ny == 64 && geo.nz == 64;
+}
+
+int launch_plane_ifft64(const double2* q, double2* T, int64_t state0,
+                        int nstates, const BoxGeom& geo, const double2* tw64,
+                        hipStream_t s) {
+  size_t shm = (size_t)(64 * kPL + 64) * sizeof(double2);
+  hipLaunchKernelGGL(plane_ifft64_kernel, dim3(geo.nxa, nstates),
+                     dim3(kPlaneBlock), shm, s, q, T, state0, geo, tw64);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+int launch_plane_fft_gather64(const double2* T, const double2* q,
+                              double2* qbar, double* kin_partial,
+                              int64_t state0, int nstates, const BoxGeom& geo,
+                              const double2* tw64, const GatherParams& gp,
+                              hipStream_t s) {
+  size_t shm = (size_t)(64 * kPL + 64) * sizeof(double2);
+  hipLaunchKernelGGL(plane_fft_gather64_kernel, dim3(geo.nxa, nstates),
+                     dim3(kPlaneBlock), shm, s, T, q, qbar, kin_partial,
+                     state0, geo, tw64, gp);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+int launch_xline_density64(const double2* T, double* rho_part, int groups,
+                           const double* occ, int64_t state0, int nstates,
+                           double inv_vol, const BoxGeom& geo,
+                           const double2* tw64, hipStream_t s) {
+  const int nzt = (geo.nz + 31) / 32;
+  hipLaunchKernelGGL(xline_density64_kernel, dim3(geo.ny * nzt, groups),
+                     dim3(kXBlock), 0, s, T, rho_part, occ, state0, nstates,
+                     inv_vol, geo, tw64);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+int launch_xline_vpsi64(double2* T, const double* v, int groups, int nstates,
+                        const BoxGeom& geo, const double2* tw64,
+                        hipStream_t s) {
+  const int nzt = (geo.nz + 31) / 32;
+  hipLaunchKernelGGL(xline_vpsi64_kernel, dim3(geo.ny * nzt, groups),
+                     dim3(kXBlock), 0, s, T, v, nstates, geo, tw64);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
 }  // namespace jrx

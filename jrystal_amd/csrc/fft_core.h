@@ -252,4 +252,35 @@ __device__ __forceinline__ void fft_lines(double2* buf, int nlines, int ls,
     lds_fft<INV, BLOCK, CAP, LINE_FAST>(buf, nlines, ls, es, lmap, L);
 }
 
+// ------------------------------------------- register 64-point transforms --
+// Eight lanes own one line of 64 points; lane j holds x = j + 8m in v[m]
+// (m = 0..7) on entry and X[j + 8t] in v[t] on exit:
+//   Y_j[a] = sum_m x[j+8m] w8^{ma};  Y_j[a] *= w64^{ja};
+//   exchange through 64 LDS slots of the line (slot(a, j) = 8a + (j+a)%8,
+//   bank-swizzled);  X[b + 8t] = sum_j Y_j[b] w8^{jt} on lane b.
+// Slots are base[s * es].  The lanes of a line are in one wave, so the
+// exchange needs only wave-level ordering.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool INV>
+__device__ __forceinline__ void fft64_8x8(double2 (&v)[8], double2* base,
+                                          int es, int j,
+                                          const double2* __restrict__ tw) {
+  Dft<8, INV>::run(v, nullptr, 0);
+#pragma unroll
+  for (int a = 1; a < 8; ++a) v[a] = cmul(v[a], twid<INV>(tw, j * a));
+  wave_sync();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) base[(8 * a + ((j + a) & 7)) * es] = v[a];
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = base[(8 * j + ((i + j) & 7)) * es];
+  wave_sync();
+  Dft<8, INV>::run(v, nullptr, 0);
+}
+
 }  // namespace jrx

@@ -16,6 +16,7 @@ struct BoxGeom {
   const short* rows;    // active (y) rows of each plane, concatenated
   const int* row_off;   // [nxa+1] offsets into rows
   int max_rows;         // max active rows over planes
+  const int* xmap;      // [nx]    active-plane index of x, or -1
 };
 
 // y = scale * DFT_axis(x) along one axis of a (B, n0, n1, n2) complex box.
@@ -69,6 +70,25 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
                             const BoxGeom& geo, const FftLen& Ly,
                             const FftLen& Lz, const GatherParams& gp,
                             hipStream_t s);
+
+// Register-resident 64-point kernels (nx = ny = nz = 64); tw64 is the
+// length-64 twiddle table.  Same contracts as the generic launchers.
+bool use_reg64(const BoxGeom& geo);
+int launch_plane_ifft64(const double2* q, double2* T, int64_t state0,
+                        int nstates, const BoxGeom& geo, const double2* tw64,
+                        hipStream_t s);
+int launch_plane_fft_gather64(const double2* T, const double2* q,
+                              double2* qbar, double* kin_partial,
+                              int64_t state0, int nstates, const BoxGeom& geo,
+                              const double2* tw64, const GatherParams& gp,
+                              hipStream_t s);
+int launch_xline_density64(const double2* T, double* rho_part, int groups,
+                           const double* occ, int64_t state0, int nstates,
+                           double inv_vol, const BoxGeom& geo,
+                           const double2* tw64, hipStream_t s);
+int launch_xline_vpsi64(double2* T, const double* v, int groups, int nstates,
+                        const BoxGeom& geo, const double2* tw64,
+                        hipStream_t s);
 
 // Largest plane (ny*nz) the fused plane kernels hold in LDS.
 constexpr int kPlaneCap = 6400;

@@ -1,6 +1,7 @@
 // Plan, workspace layout and orchestration of one energy(+gradient)
 // evaluation; C ABI entry points.
 #include <math.h>
+#include <stdlib.h>
 
 #include <map>
 #include <mutex>
@@ -78,6 +79,8 @@ struct jrx_plan {
   double* d_kpts = nullptr;
   double* d_occ = nullptr;
   int chunk = 0;   // states per FFT chunk (never straddles spins)
+  bool force_generic = false;  // run the runtime-radix kernels at 64^3
+  bool keep_planes = false;    // plane buffer holds all states
   int groups = 1;  // state groups of the x-line kernels
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
@@ -199,6 +202,8 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   JRX_REQUIRE(nx <= 2047 && ny <= 2047 && nz <= 2047, "grid too large");
   plane_g0.push_back(ng);
   row_off.push_back((int)rows.size());
+  std::vector<int> xmap(nx, -1);
+  for (size_t i = 0; i < plane_x.size(); ++i) xmap[plane_x[i]] = (int)i;
   p->ng = ng;
   p->geo.nx = nx; p->geo.ny = ny; p->geo.nz = nz; p->geo.ng = ng;
   p->geo.nxa = (int)plane_x.size();
@@ -217,6 +222,8 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   if ((rc = upload(p, rows.data(), rows.size(), &srows))) return rc;
   p->geo.rows = srows;
   if ((rc = upload(p, g_m.data(), g_m.size(), &p->d_g_m))) return rc;
+  if ((rc = upload(p, xmap.data(), xmap.size(), &tmp))) return rc;
+  p->geo.xmap = tmp;
   if ((rc = make_len(p, nx, &p->Lx))) return rc;
   if ((rc = make_len(p, ny, &p->Ly))) return rc;
   if ((rc = make_len(p, nz, &p->Lz))) return rc;
@@ -251,17 +258,27 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   }
   // Chunking of the FFT sweeps.
   const size_t per_state = (size_t)p->geo.nxa * ny * nz * sizeof(double2);
+  // Chunking of the FFT sweeps.  When the plane buffer can hold every state
+  // (default budget 16 GiB), the forward sweep's planes are kept and the
+  // adjoint sweep starts from them instead of transforming Q again.
   int64_t chunk = d->chunk_states;
   if (chunk <= 0) {
-    const size_t budget = (size_t)1 << 30;
+    const size_t budget = (size_t)16 << 30;
     chunk = (int64_t)(budget / per_state);
     if (chunk < 1) chunk = 1;
   }
   const int64_t spin_states = (int64_t)d->nk * d->nb;
-  if (chunk > spin_states) chunk = spin_states;
+  p->keep_planes = chunk >= p->S && p->S <= 65535;
+  if (p->keep_planes) chunk = p->S;
+  else if (chunk > spin_states) chunk = spin_states;
   if (chunk > 65535) chunk = 65535;
   p->chunk = (int)chunk;
-  p->groups = xline_groups(p->geo, p->chunk);
+  // JRX_FFT_GENERIC=1 selects the runtime-radix FFT kernels even where the
+  // register-resident 64-point ones apply (used to cross-check them).
+  const char* gen = getenv("JRX_FFT_GENERIC");
+  p->force_generic = gen && gen[0] == '1';
+  p->groups = xline_groups(p->geo, p->keep_planes ? (int)spin_states
+                                                   : p->chunk);
   p->gsw = gram_split(p->P, ng);
   p->gsq = p->gsw;
   const int64_t kin_gather = p->S * p->geo.nxa;
@@ -363,6 +380,7 @@ int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info) {
   info->ng = plan->ng;
   info->active_planes = plan->geo.nxa;
   info->chunk_states = plan->chunk;
+  info->keep_planes = plan->keep_planes ? 1 : 0;
   info->workspace_bytes = plan->total;
   return JRX_OK;
 }
@@ -466,17 +484,30 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   JRX_T(JRX_K_QR_APPLY, launch_apply_fwd(w_re, w_im, Ri, q, P, ng, nb, s));
   // --- density -------------------------------------------------------------
   const int64_t SS = (int64_t)p->nk * nb;  // states per spin
+  const bool reg64 = use_reg64(p->geo) && !p->force_generic;
+  const int64_t plane_elems = (int64_t)p->geo.nxa * p->ny * p->nz;
   for (int sp = 0; sp < p->ns; ++sp) {
     JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
-    for (int64_t s0 = 0; s0 < SS; s0 += p->chunk) {
-      const int nst = (int)((SS - s0) < p->chunk ? (SS - s0) : p->chunk);
+    const int64_t step = p->keep_planes ? SS : p->chunk;
+    for (int64_t s0 = 0; s0 < SS; s0 += step) {
+      const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
       const int64_t st0 = sp * SS + s0;
-      JRX_T(JRX_K_PLANE_IFFT,
-            launch_plane_ifft(q, T, st0, nst, p->geo, p->Ly, p->Lz, s));
-      JRX_T(JRX_K_XLINE_DENSITY,
-            launch_xline_density(T, rpart, xline_groups(p->geo, nst),
-                                 p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
-                                 p->Lx, s));
+      double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
+      if (reg64) {
+        JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst, p->geo,
+                                                    p->Lx.tw, s));
+        JRX_T(JRX_K_XLINE_DENSITY,
+              launch_xline_density64(Tc, rpart, xline_groups(p->geo, nst),
+                                     p->d_occ, st0, nst, 1.0 / p->vol,
+                                     p->geo, p->Lx.tw, s));
+      } else {
+        JRX_T(JRX_K_PLANE_IFFT,
+              launch_plane_ifft(q, Tc, st0, nst, p->geo, p->Ly, p->Lz, s));
+        JRX_T(JRX_K_XLINE_DENSITY,
+              launch_xline_density(Tc, rpart, xline_groups(p->geo, nst),
+                                   p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
+                                   p->Lx, s));
+      }
     }
     JRX_T(JRX_K_XLINE_DENSITY,
           launch_reduce_groups(rpart, rho + sp * p->N, p->groups, p->N, s));
@@ -500,18 +531,34 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
     gp.nk = p->nk;
     gp.nb = nb;
     gp.two_over_vol = 2.0 / p->vol;
+    const int64_t step = p->keep_planes ? SS : p->chunk;
     for (int sp = 0; sp < p->ns; ++sp) {
-      for (int64_t s0 = 0; s0 < SS; s0 += p->chunk) {
-        const int nst = (int)((SS - s0) < p->chunk ? (SS - s0) : p->chunk);
+      for (int64_t s0 = 0; s0 < SS; s0 += step) {
+        const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
         const int64_t st0 = sp * SS + s0;
-        JRX_T(JRX_K_PLANE_IFFT,
-              launch_plane_ifft(q, T, st0, nst, p->geo, p->Ly, p->Lz, s));
-        JRX_T(JRX_K_XLINE_VPSI,
-              launch_xline_vpsi(T, v + sp * p->N, xline_groups(p->geo, nst),
-                                nst, p->geo, p->Lx, s));
-        JRX_T(JRX_K_PLANE_FFT_GATHER,
-              launch_plane_fft_gather(T, q, qbar, kin, st0, nst, p->geo,
-                                      p->Ly, p->Lz, gp, s));
+        double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
+        if (reg64) {
+          if (!p->keep_planes)
+            JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst,
+                                                        p->geo, p->Lx.tw, s));
+          JRX_T(JRX_K_XLINE_VPSI,
+                launch_xline_vpsi64(Tc, v + sp * p->N,
+                                    xline_groups(p->geo, nst), nst, p->geo,
+                                    p->Lx.tw, s));
+          JRX_T(JRX_K_PLANE_FFT_GATHER,
+                launch_plane_fft_gather64(Tc, q, qbar, kin, st0, nst, p->geo,
+                                          p->Lx.tw, gp, s));
+        } else {
+          if (!p->keep_planes)
+            JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, Tc, st0, nst, p->geo,
+                                                      p->Ly, p->Lz, s));
+          JRX_T(JRX_K_XLINE_VPSI,
+                launch_xline_vpsi(Tc, v + sp * p->N, xline_groups(p->geo, nst),
+                                  nst, p->geo, p->Lx, s));
+          JRX_T(JRX_K_PLANE_FFT_GATHER,
+                launch_plane_fft_gather(Tc, q, qbar, kin, st0, nst, p->geo,
+                                        p->Ly, p->Lz, gp, s));
+        }
       }
     }
     nkin = (int)(S * p->geo.nxa);

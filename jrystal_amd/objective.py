@@ -86,6 +86,7 @@ class EnergyPlan:
     self.num_g = info.ng
     self.active_planes = info.active_planes
     self.chunk_states = info.chunk_states
+    self.keep_planes = bool(info.keep_planes)
     self.ns, self.nk, self.nb = occ.shape
     self.grid_sizes = mask.shape
     self.param_shape = (self.ns, self.nk, self.num_g, self.nb)

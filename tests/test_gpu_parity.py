@@ -163,3 +163,17 @@ def test_golden_vectors(name):
   for part, n in (("re", norm[0]), ("im", norm[1])):
     got = out[f"dw_{part}"][t]
     assert np.max(np.abs(got - gold[f"{name}/grad_{part}"])) < G_TOL * n
+
+
+def test_register_fft64_matches_generic(monkeypatch):
+  """64^3 register-resident kernels against the runtime-radix kernels."""
+  from jrystal_amd import workloads as W
+  wl = W.si8_cube(4.0, k=2)
+  params = seeded_params(wl.shape, 9)
+  a = _run(_plan(wl.crystal, wl.mask, wl.kpts, wl.occupation), params)
+  monkeypatch.setenv("JRX_FFT_GENERIC", "1")
+  b = _run(_plan(wl.crystal, wl.mask, wl.kpts, wl.occupation), params)
+  assert np.max(np.abs(a["energies"] - b["energies"])) < 1e-10
+  assert np.max(np.abs(a["density"] - b["density"])) < 1e-13
+  assert rel_max(a["dw_re"], b["dw_re"]) < 1e-12
+  assert rel_max(a["dw_im"], b["dw_im"]) < 1e-12
