@@ -96,6 +96,36 @@ int jrx_energy_grad(jrx_plan* plan, const double* w_re, const double* w_im,
                     double* rho, double* q, void* workspace,
                     size_t workspace_bytes, void* stream);
 
+/* Preconditioned minimisation support (no counterpart in the reference,
+ * whose driver runs optax Adam on the raw parameters).  After
+ * jrx_energy_grad the workspace holds, per state, tb = sum_G |G+k|^2 |q|^2
+ * (twice the band kinetic energy).
+ * jrx_plan_precondition: out = K(|G+k|^2 / tb) * g (Teter-Payne-Allan
+ *   1989), times factor[i] when factor is not NULL; g, out, factor in the
+ *   [s][k][g][b] parameter layout.
+ * jrx_plan_band_norms: copies tb [s][k][b] to out.
+ * jrx_plan_orthonormal_params: writes Q of the last evaluation (W = QR,
+ *   R upper triangular with positive diagonal) in the [s][k][g][b]
+ *   parameter layout; the energy is invariant under W -> Q, so a minimiser
+ *   may reset its parameters to Q to undo drift of the gauge R. */
+int jrx_plan_precondition(jrx_plan* plan, const double* g_re,
+                          const double* g_im, const double* factor,
+                          double* out_re, double* out_im, void* workspace,
+                          size_t workspace_bytes, void* stream);
+int jrx_plan_band_norms(const jrx_plan* plan, double* out,
+                        const void* workspace, size_t workspace_bytes,
+                        void* stream);
+int jrx_plan_orthonormal_params(const jrx_plan* plan, double* out_re,
+                                double* out_im, const void* workspace,
+                                size_t workspace_bytes, void* stream);
+/* Flat float64 vector helpers: out[0] = sum a*b (deterministic, device
+ * scalar); out = alpha x + beta y (out may be y). */
+size_t jrx_dot_workspace_bytes(void);
+int jrx_vec_dot(const double* a, const double* b, int64_t n, double* out,
+                void* workspace, size_t workspace_bytes, void* stream);
+int jrx_vec_axpby(double alpha, const double* x, double beta, const double* y,
+                  double* out, int64_t n, void* stream);
+
 /* ------------------------------------------------------------------------
  * Stand-alone operators behind the reference's public functions (full-box
  * layout).  Complex arrays interleaved; "state" = (spin, kpt, band) flattened

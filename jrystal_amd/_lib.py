@@ -74,6 +74,12 @@ SIGNATURES = {
   "jrx_scale": [_P, _P, _P, _I64, _P],
   "jrx_ewald_workspace_bytes": [_I64, _I64],
   "jrx_ewald": [_P, _P, _I, _P, _I64, _D, _D, _P, _I64, _P, _P, _SZ, _P],
+  "jrx_plan_precondition": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_band_norms": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_orthonormal_params": [_P, _P, _P, _P, _SZ, _P],
+  "jrx_dot_workspace_bytes": [],
+  "jrx_vec_dot": [_P, _P, _I64, _P, _P, _SZ, _P],
+  "jrx_vec_axpby": [_D, _P, _D, _P, _P, _I64, _P],
   "jrx_plan_set_timing": [_P, _I],
   "jrx_plan_read_timing": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
 }
@@ -89,7 +95,8 @@ _RESTYPES = {"jrx_last_error": ctypes.c_char_p,
              "jrx_qr_workspace_bytes": ctypes.c_size_t,
              "jrx_kinetic_workspace_bytes": ctypes.c_size_t,
              "jrx_reduce_workspace_bytes": ctypes.c_size_t,
-             "jrx_ewald_workspace_bytes": ctypes.c_size_t}
+             "jrx_ewald_workspace_bytes": ctypes.c_size_t,
+             "jrx_dot_workspace_bytes": ctypes.c_size_t}
 
 
 def lib():

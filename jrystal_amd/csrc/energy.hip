@@ -159,7 +159,8 @@ __global__ __launch_bounds__(kRedBlock) void kinetic_kernel(
     const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7] + kpts[3 * k + 1];
     const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8] + kpts[3 * k + 2];
     const double2 c = q[st * ng + g];
-    kin = occ[st] * (gx * gx + gy * gy + gz * gz) * (c.x * c.x + c.y * c.y);
+    kin = (gx * gx + gy * gy + gz * gz) * (c.x * c.x + c.y * c.y);
+    (void)occ;
   }
   const double t = block_sum(kin, red);
   if (threadIdx.x == 0) kin_part[st * gblocks + gbk] = t;
@@ -188,11 +189,34 @@ __device__ double strided_sum(const double* p, int n, double* red) {
   return block_sum(acc, red);
 }
 
+// tb[st] = sum_j part[st * nper + j] (fixed order): |G+k|^2-weighted norm
+// of each state, i.e. twice its kinetic energy.
+__global__ __launch_bounds__(kRedBlock) void state_sums_kernel(
+    const double* __restrict__ part, int nper, int64_t S,
+    double* __restrict__ tb) {
+  const int64_t st = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
+  if (st >= S) return;
+  double a = 0.0;
+  for (int j = 0; j < nper; ++j) a += part[st * nper + j];
+  tb[st] = a;
+}
+
+int launch_state_sums(const double* part, int nper, int64_t S, double* tb,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(state_sums_kernel, dim3(ceil_div(S, kRedBlock)),
+                     dim3(kRedBlock), 0, s, part, nper, S, tb);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
 __global__ __launch_bounds__(kRedBlock) void finalize_energies_kernel(
-    const double* kin, int nkin, const double* eh, const double* ee, int nrec,
-    const double* exc, int nexc, double* energies) {
+    const double* tb, const double* occ, int64_t S, const double* eh,
+    const double* ee, int nrec, const double* exc, int nexc,
+    double* energies) {
   __shared__ double red[kRedBlock];
-  const double k = 0.5 * strided_sum(kin, nkin, red);
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < S; i += kRedBlock) a += occ[i] * tb[i];
+  const double k = 0.5 * block_sum(a, red);
   const double h = strided_sum(eh, nrec, red);
   const double e = strided_sum(ee, nrec, red);
   const double x = strided_sum(exc, nexc, red);
@@ -205,11 +229,12 @@ __global__ __launch_bounds__(kRedBlock) void finalize_energies_kernel(
   }
 }
 
-int launch_finalize_energies(const double* kin, int nkin, const double* eh,
-                             const double* ee, int nrec, const double* exc,
-                             int nexc, double* energies, hipStream_t s) {
+int launch_finalize_energies(const double* tb, const double* occ, int64_t S,
+                             const double* eh, const double* ee, int nrec,
+                             const double* exc, int nexc, double* energies,
+                             hipStream_t s) {
   hipLaunchKernelGGL(finalize_energies_kernel, dim3(1), dim3(kRedBlock), 0, s,
-                     kin, nkin, eh, ee, nrec, exc, nexc, energies);
+                     tb, occ, S, eh, ee, nrec, exc, nexc, energies);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

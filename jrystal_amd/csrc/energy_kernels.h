@@ -34,17 +34,23 @@ int launch_potential_real(const double2* vbox, const double* rho, double* v,
                           double* exc_part, int ns, int64_t N, double vol,
                           double cx, int* nparts, hipStream_t s);
 
-// Kinetic partials f |G+k|^2 |q|^2 per (state, g-block), band-major q.
+// Partials of |G+k|^2 |q|^2 per (state, g-block), band-major q.
 int launch_kinetic(const double2* q, const double* occ, const double* kpts,
                    const short* g_m, const CellParams& cp, int64_t nstates,
                    int ng, int nk, int nb, double* kin_part, int* nparts,
                    hipStream_t s);
 
-// energies[0..4] = kinetic, hartree, external, xc, total from the partials
-// (fixed summation order).
-int launch_finalize_energies(const double* kin, int nkin, const double* eh,
-                             const double* ee, int nrec, const double* exc,
-                             int nexc, double* energies, hipStream_t s);
+// tb[st] = sum_j part[st * nper + j]: sum_G |G+k|^2 |q|^2 of each state
+// (twice the band kinetic energy).
+int launch_state_sums(const double* part, int nper, int64_t S, double* tb,
+                      hipStream_t s);
+
+// energies[0..4] = kinetic (1/2 sum occ tb), hartree, external, xc, total
+// from the partials (fixed summation order).
+int launch_finalize_energies(const double* tb, const double* occ, int64_t S,
+                             const double* eh, const double* ee, int nrec,
+                             const double* exc, int nexc, double* energies,
+                             hipStream_t s);
 
 int launch_fill(double* p, double v, int64_t n, hipStream_t s);
 

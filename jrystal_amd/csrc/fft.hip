@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
     const double2 hv = sm[yz];
     qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
                          f * (gp.two_over_vol * hv.y + gk2 * cv.y));
-    kin += f * gk2 * (cv.x * cv.x + cv.y * cv.y);
+    kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
   }
   __syncthreads();  // the plane is dead: reuse its LDS for the reduction
   const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather64_kernel(
     const double2 hv = pl[iy * kPL + iz];
     qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
                          f * (gp.two_over_vol * hv.y + gk2 * cv.y));
-    kin += f * gk2 * (cv.x * cv.x + cv.y * cv.y);
+    kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
   }
   __syncthreads();
   const double tot = block_sum(kin, reinterpret_cast<double*>(sm));

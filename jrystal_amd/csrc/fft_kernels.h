@@ -57,7 +57,7 @@ int launch_xline_vpsi(double2* T, const double* v, int groups, int nstates,
                       const BoxGeom& geo, const FftLen& Lx, hipStream_t s);
 
 // qbar[state][g] = f (2/vol * FFT_yz(T)[g] + |G+k|^2 q[state][g]) and the
-// per-(state, plane) kinetic partials f |G+k|^2 |q|^2 (summed later).
+// per-(state, plane) partials of |G+k|^2 |q|^2 (summed per state later).
 struct GatherParams {
   const double* occ;   // [states]
   const double* kpts;  // [nk*3]

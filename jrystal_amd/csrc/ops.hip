@@ -354,6 +354,29 @@ __global__ void adam_kernel(double* __restrict__ p, const double* __restrict__ g
   p[i] -= lr * (mi / c1) / (sqrt(vi / c2 + eps_root) + eps);
 }
 
+// Deterministic dot product: block b sums the contiguous slice
+// [b*chunk, (b+1)*chunk) (fixed thread order and tree), then one block sums
+// the block partials.
+__global__ void vec_dot_part_kernel(const double* __restrict__ a,
+                                    const double* __restrict__ b, int64_t n,
+                                    int64_t chunk, double* __restrict__ part) {
+  __shared__ double red[kB];
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  double acc = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kB) acc += a[i] * b[i];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// y and out may be the same buffer (in-place update).
+__global__ void axpby_kernel(double alpha, const double* __restrict__ x,
+                             double beta, const double* y, double* out,
+                             int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n) out[i] = alpha * x[i] + beta * y[i];
+}
+
 inline dim3 grid2(int64_t n, int64_t y) {
   return dim3((unsigned)ceil_div(n, kB), (unsigned)y);
 }
@@ -382,6 +405,40 @@ int jrx_adam_update(double* param, const double* grad, double* m, double* v,
   hipLaunchKernelGGL(adam_kernel, dim3(ceil_div(n, kB)), dim3(kB), 0,
                      (hipStream_t)stream, param, grad, m, v, n, lr, b1, b2,
                      eps, eps_root, c1, c2);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+constexpr int kDotBlocks = 1024;
+
+size_t jrx_dot_workspace_bytes(void) { return (kDotBlocks + 8) * sizeof(double); }
+
+int jrx_vec_dot(const double* a, const double* b, int64_t n, double* out,
+                void* workspace, size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(a && b && out && workspace && n >= 0, "invalid dot arguments");
+  JRX_REQUIRE(workspace_bytes >= jrx_dot_workspace_bytes(),
+              "dot workspace too small");
+  int64_t nb = ceil_div(n, (int64_t)kB * 16);
+  if (nb < 1) nb = 1;
+  if (nb > kDotBlocks) nb = kDotBlocks;
+  const int64_t chunk = (n + nb - 1) / nb;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)workspace;
+  hipLaunchKernelGGL(vec_dot_part_kernel, dim3((unsigned)nb), dim3(kB), 0, s,
+                     a, b, n, chunk > 0 ? chunk : 1, part);
+  JRX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(kB), 0, s, part, nullptr, nb,
+                     1.0, out);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+int jrx_vec_axpby(double alpha, const double* x, double beta, const double* y,
+                  double* out, int64_t n, void* stream) {
+  JRX_REQUIRE(x && y && out && n >= 0, "invalid axpby arguments");
+  if (n == 0) return JRX_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3(ceil_div(n, kB)), dim3(kB), 0,
+                     (hipStream_t)stream, alpha, x, beta, y, out, n);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

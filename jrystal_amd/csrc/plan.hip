@@ -89,7 +89,7 @@ struct jrx_plan {
   std::vector<hipEvent_t> events;
   std::vector<std::pair<int, int>> records;  // (class, event pair index)
   // workspace offsets (bytes)
-  size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_S, o_R1, o_R2, o_Ri, o_A,
+  size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_tb, o_S, o_R1, o_R2, o_Ri, o_A,
       o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, total;
 };
 
@@ -308,6 +308,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_Bm = take(mat);
   p->o_part = take((size_t)p->P * p->gsw.nchunk * 4 * nb * nb * sizeof(double2));
   p->o_kin = take((size_t)p->kin_parts_max * sizeof(double));
+  p->o_tb = take((size_t)p->S * sizeof(double));
   p->o_eh = take((size_t)p->rec_parts * sizeof(double));
   p->o_ee = take((size_t)p->rec_parts * sizeof(double));
   p->o_exc = take((size_t)p->rec_parts * sizeof(double));
@@ -382,6 +383,112 @@ int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info) {
   info->chunk_states = plan->chunk;
   info->keep_planes = plan->keep_planes ? 1 : 0;
   info->workspace_bytes = plan->total;
+  return JRX_OK;
+}
+
+// Teter-Payne-Allan diagonal preconditioner (PRB 40, 12255 (1989)) applied
+// to a gradient in the parameter layout [s][k][g][b]:
+//   x = |G+k|^2 / tb[state]  (plane-wave over band kinetic energy),
+//   K(x) = (27 + 18x + 12x^2 + 8x^3) / (27 + 18x + 12x^2 + 8x^3 + 16x^4),
+// optionally times a per-coefficient factor (learned preconditioner).
+__global__ void tpa_kernel(const double* __restrict__ gr,
+                           const double* __restrict__ gi,
+                           const double* __restrict__ tb,
+                           const short* __restrict__ g_m,
+                           const double* __restrict__ kpts, CellParams cp,
+                           int ng, int nk, int nb, int64_t n,
+                           const double* __restrict__ factor,
+                           double* __restrict__ outr,
+                           double* __restrict__ outi) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)(i % nb);
+  const int64_t rest = i / nb;
+  const int g = (int)(rest % ng);
+  const int64_t sk = rest / ng;
+  const int k = (int)(sk % nk);
+  const int64_t st = sk * nb + b;
+  const int mx = g_m[3 * g], my = g_m[3 * g + 1], mz = g_m[3 * g + 2];
+  const double gx = mx * cp.B[0] + my * cp.B[3] + mz * cp.B[6] + kpts[3 * k];
+  const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7] + kpts[3 * k + 1];
+  const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8] + kpts[3 * k + 2];
+  const double t = tb[st];
+  const double x = (gx * gx + gy * gy + gz * gz) / (t > 1e-300 ? t : 1e-300);
+  const double p = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+  double kx = p / (p + 16.0 * x * x * x * x);
+  if (factor) kx *= factor[i];
+  outr[i] = kx * gr[i];
+  outi[i] = kx * gi[i];
+}
+
+extern "C" int jrx_plan_precondition(jrx_plan* p, const double* g_re,
+                                     const double* g_im, const double* factor,
+                                     double* out_re, double* out_im,
+                                     void* workspace, size_t workspace_bytes,
+                                     void* stream) {
+  JRX_REQUIRE(p && g_re && g_im && out_re && out_im && workspace,
+              "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  const int64_t n = p->S * p->ng;
+  const double* tb = reinterpret_cast<const double*>(
+      static_cast<const char*>(workspace) + p->o_tb);
+  hipLaunchKernelGGL(tpa_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, g_re, g_im, tb, p->d_g_m, p->d_kpts,
+                     p->cp, p->ng, p->nk, p->nb, n, factor, out_re, out_im);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+extern "C" int jrx_plan_band_norms(const jrx_plan* p, double* out,
+                                   const void* workspace,
+                                   size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && out && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  JRX_CHECK_HIP(hipMemcpyAsync(
+      out, static_cast<const char*>(workspace) + p->o_tb,
+      (size_t)p->S * sizeof(double), hipMemcpyDeviceToDevice,
+      (hipStream_t)stream));
+  return JRX_OK;
+}
+
+// Q [P][nb][ng] (complex, band-major) -> out_re / out_im [P][ng][nb], one
+// 64-wide G tile of one (s,k) block per workgroup, transposed through LDS.
+__global__ void __launch_bounds__(256)
+q_to_params_kernel(const double2* __restrict__ q, int ng, int nb,
+                   double* __restrict__ outr, double* __restrict__ outi) {
+  __shared__ double2 tile[kMaxBands][65];
+  const int64_t pk = blockIdx.y;
+  const int g0 = blockIdx.x * 64;
+  const double2* src = q + pk * (int64_t)nb * ng;
+  for (int i = threadIdx.x; i < nb * 64; i += 256) {
+    const int b = i >> 6, gl = i & 63, g = g0 + gl;
+    tile[b][gl] = g < ng ? src[(int64_t)b * ng + g] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb * 64; i += 256) {
+    const int gl = i / nb, b = i % nb, g = g0 + gl;
+    if (g < ng) {
+      const int64_t o = (pk * ng + g) * nb + b;
+      outr[o] = tile[b][gl].x;
+      outi[o] = tile[b][gl].y;
+    }
+  }
+}
+
+extern "C" int jrx_plan_orthonormal_params(const jrx_plan* p, double* out_re,
+                                           double* out_im,
+                                           const void* workspace,
+                                           size_t workspace_bytes,
+                                           void* stream) {
+  JRX_REQUIRE(p && out_re && out_im && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  const double2* q = reinterpret_cast<const double2*>(
+      static_cast<const char*>(workspace) + p->o_q);
+  hipLaunchKernelGGL(q_to_params_kernel,
+                     dim3((unsigned)ceil_div(p->ng, 64), (unsigned)p->P),
+                     dim3(256), 0, (hipStream_t)stream, q, p->ng, p->nb,
+                     out_re, out_im);
+  JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
 
@@ -460,6 +567,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   double2* Bm = cptr(p->o_Bm);
   double2* part = cptr(p->o_part);
   double* kin = dptr(p->o_kin);
+  double* tb = dptr(p->o_tb);
   double* eh = dptr(p->o_eh);
   double* ee = dptr(p->o_ee);
   double* exc = dptr(p->o_exc);
@@ -561,7 +669,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
         }
       }
     }
-    nkin = (int)(S * p->geo.nxa);
+    JRX_T(JRX_K_OTHER, launch_state_sums(kin, p->geo.nxa, S, tb, s));
     // --- QR adjoint ----------------------------------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
     JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsq.nchunk, nb, s));
@@ -570,9 +678,10 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   } else {
     JRX_T(JRX_K_OTHER, launch_kinetic(q, p->d_occ, p->d_kpts, p->d_g_m, p->cp, S, ng,
                            p->nk, nb, kin, &nkin, s));
+    JRX_T(JRX_K_OTHER, launch_state_sums(kin, (int)(nkin / S), S, tb, s));
   }
-  JRX_T(JRX_K_OTHER, launch_finalize_energies(kin, nkin, eh, ee, nrec, exc, nexc,
-                                   energies, s));
+  JRX_T(JRX_K_OTHER, launch_finalize_energies(tb, p->d_occ, S, eh, ee, nrec,
+                                              exc, nexc, energies, s));
   if (rho_out)
     JRX_CHECK_HIP(hipMemcpyAsync(rho_out, rho,
                                  (size_t)p->ns * p->N * sizeof(double),

@@ -157,6 +157,51 @@ class EnergyPlan:
     _lib.check(rc)
     return out
 
+  def precondition(self, g_re, g_im, factor=None, out_re=None, out_im=None):
+    """Kinetic (Teter-Payne-Allan) preconditioning of a gradient:
+    out = K(|G+k|^2 / tb) * factor * g, with the band norms tb of the last
+    evaluation on this plan (K(x) = p / (p + 16x^4),
+    p = 27 + 18x + 12x^2 + 8x^3).  ``factor`` is an optional (s,k,g,b)
+    float64 tensor of per-coefficient multipliers."""
+    g_re = self._as_param(g_re, "g_re")
+    g_im = self._as_param(g_im, "g_im")
+    if factor is not None:
+      factor = self._as_param(factor, "factor")
+    out_re = torch.empty_like(g_re) if out_re is None else out_re
+    out_im = torch.empty_like(g_im) if out_im is None else out_im
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_precondition(
+        self._plan, _lib.ptr(g_re), _lib.ptr(g_im), _lib.ptr(factor),
+        _lib.ptr(out_re), _lib.ptr(out_im), self._ws_ptr,
+        self.workspace_bytes, _lib.stream_ptr(self.device)))
+    return out_re, out_im
+
+  def band_norms(self, out=None):
+    """tb[s,k,b] = sum_G |G+k|^2 |Q_Gb|^2 of the last evaluation (twice the
+    kinetic energy of each band), device tensor (ns,nk,nb)."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb), dtype=torch.float64,
+                        device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_band_norms(
+        self._plan, _lib.ptr(out), self._ws_ptr, self.workspace_bytes,
+        _lib.stream_ptr(self.device)))
+    return out
+
+  def orthonormal_params(self, out_re=None, out_im=None):
+    """Q of the last evaluation in the (s,k,g,b) parameter layout (real and
+    imaginary parts).  E(Q) = E(W), and Q has R = I."""
+    shape = self.param_shape
+    if out_re is None:
+      out_re = torch.empty(shape, dtype=torch.float64, device=self.device)
+    if out_im is None:
+      out_im = torch.empty(shape, dtype=torch.float64, device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_orthonormal_params(
+        self._plan, _lib.ptr(out_re), _lib.ptr(out_im), self._ws_ptr,
+        self.workspace_bytes, _lib.stream_ptr(self.device)))
+    return out_re, out_im
+
   def set_timing(self, enable: bool):
     _lib.check(self._lib.jrx_plan_set_timing(self._plan, 1 if enable else 0))
 

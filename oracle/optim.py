@@ -50,3 +50,36 @@ def converged(history, window, threshold):
   if len(history) < window:
     return False
   return bool(np.std(history[-window:]) < threshold)
+
+
+def tpa_factor(x):
+  """Teter-Payne-Allan kinetic preconditioner (PRB 40, 12255 (1989)):
+  K(x) = p / (p + 16 x^4), p = 27 + 18x + 12x^2 + 8x^3, with
+  x = (kinetic energy of the plane wave) / (kinetic energy of the band).
+  K = 1 - (16/27) x^4 + O(x^5) for small x and K ~ 1/(2x) for large x."""
+  x = np.asarray(x, dtype=float)
+  p = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x))
+  return p / (p + 16.0 * x**4)
+
+
+def band_norms(params, prob):
+  """tb[s,k,b] = sum_G |G+k|^2 |Q[s,k,G,b]|^2, Q from the reduced QR of
+  W = w_re + i w_im: twice the kinetic energy of each band, so that
+  E_kin = 1/2 sum occ * tb."""
+  w = params["w_re"] + 1j * params["w_im"]
+  q = np.linalg.qr(w, mode="reduced")[0]
+  gm = prob.mask_g()
+  gk2 = np.sum((gm[None] + prob.kpts[:, None, :])**2, axis=-1)  # (k,g)
+  return np.einsum("kg,skgb->skb", gk2, np.real(np.conj(q) * q))
+
+
+def tpa_precondition(grad, params, prob, factor=None):
+  """Diagonal preconditioning of a gradient dict (w_re, w_im), each
+  (s,k,g,b): P[s,k,g,b] = K(|G+k|^2 / tb[s,k,b]) * factor[s,k,g,b]."""
+  tb = band_norms(params, prob)
+  gm = prob.mask_g()
+  gk2 = np.sum((gm[None] + prob.kpts[:, None, :])**2, axis=-1)
+  kx = tpa_factor(gk2[None, :, :, None] / tb[:, :, None, :])
+  if factor is not None:
+    kx = kx * factor
+  return {k: kx * v for k, v in grad.items()}
