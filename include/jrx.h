@@ -118,6 +118,71 @@ int jrx_plan_band_norms(const jrx_plan* plan, double* out,
 int jrx_plan_orthonormal_params(const jrx_plan* plan, double* out_re,
                                 double* out_im, const void* workspace,
                                 size_t workspace_bytes, void* stream);
+/* jrx_plan_subspace_matrix: after jrx_energy_grad with a gradient, copies
+ *   Q^H Qbar [s][k][i][j] (complex, nb x nb per (s,k)); Qbar_j = 2 f_j H q_j,
+ *   so the block of bands with equal occupation f > 0 is 2 f times the
+ *   subspace Hamiltonian q_i^H H q_j.
+ * jrx_plan_rotate: out = Q V in the parameter layout, V [s][k][c][b]
+ *   complex nb x nb.  A unitary V mixing only bands of equal occupation
+ *   leaves the energy unchanged (Rayleigh-Ritz gauge of a minimiser). */
+int jrx_plan_subspace_matrix(const jrx_plan* plan, double* out,
+                             const void* workspace, size_t workspace_bytes,
+                             void* stream);
+int jrx_plan_rotate(const jrx_plan* plan, const double* v, double* out_re,
+                    double* out_im, const void* workspace,
+                    size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned preconditioner and warm start (no counterpart in the reference).
+ * A FP32 MLP, 8 feature slots -> tanh 64 -> tanh 64 -> 2 outputs, with
+ * flat parameters (jrx_mlp_num_params() floats: W1[64][8], b1[64],
+ * W2[64][64], b2[64], W3[2][64], b3[2]) and norm[16] = (mu[8], 1/sd[8])
+ * applied to the raw features first.  Raw features of coefficient
+ * (s,k,g,b): log1p(|G+k|^2), log1p(|G+k|^2 / tb), log(tb),
+ * log(rho_mean), log1p(rho_var / rho_mean^2), 0, 0, 0 with tb the band
+ * norm and rho_mean / rho_var the statistics of the total density of the
+ * last evaluation.  Output 0 is the log preconditioner factor (clamped to
+ * [-8, 8]), output 1 the log warm-start amplitude (clamped to [-30, 8]).
+ * ---------------------------------------------------------------------- */
+/* out[0] = mean, out[1] = variance of the total density over the grid. */
+int jrx_plan_density_stats(const jrx_plan* plan, double* out,
+                           void* workspace, size_t workspace_bytes,
+                           void* stream);
+/* x[i][0..7] = raw features of coefficient idx[i] (device int64 flat
+ * [s][k][g][b] indices; out-of-range indices give zeros). */
+int jrx_plan_features(const jrx_plan* plan, const int64_t* idx, int64_t n,
+                      const double* stats, float* x, const void* workspace,
+                      size_t workspace_bytes, void* stream);
+/* Inference on every coefficient: factor = exp(y0) (if factor is not
+ * NULL), w *= exp(y1) (if w_re / w_im are not NULL). */
+int jrx_plan_mlp_apply(const jrx_plan* plan, const float* params,
+                       const float* norm, const double* stats,
+                       double* factor, double* w_re, double* w_im,
+                       const void* workspace, size_t workspace_bytes,
+                       void* stream);
+/* Plain network kernels (training).  x [n][8] raw features, y [n][2]. */
+size_t jrx_mlp_num_params(void);
+int jrx_mlp_forward(const float* params, const float* norm, const float* x,
+                    int64_t n, float* y, void* stream);
+/* grad (+)= d(sum_i dy_i . y_i)/dparams (accumulate != 0 adds). */
+size_t jrx_mlp_backward_workspace_bytes(int64_t n);
+int jrx_mlp_backward(const float* params, const float* norm, const float* x,
+                     const float* dy, int64_t n, float* grad, int accumulate,
+                     void* workspace, size_t workspace_bytes, void* stream);
+/* Snapshot loss: nsnap groups of snap_size samples; with p = exp(y0):
+ * loss[s] = 1 - (sum p a)^2 / (t2[s] sum p^2 b); dy = d(mean_s loss)/dy. */
+int jrx_mlp_cosine_loss(const float* y, const float* a, const float* b,
+                        const float* t2, int64_t snap_size, int64_t nsnap,
+                        float* dy, float* loss, void* stream);
+/* loss[0] = mean (y1 - target)^2; dy its gradient (output 0 gets 0). */
+size_t jrx_mlp_mse_workspace_bytes(int64_t n);
+int jrx_mlp_mse_loss(const float* y, const float* target, int64_t n,
+                     float* dy, float* loss, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/* Adam step (optax rule, FP32); count = steps taken including this one. */
+int jrx_mlp_adam(float* params, const float* grad, float* m, float* v,
+                 int64_t n, double lr, double b1, double b2, double eps,
+                 int64_t count, void* stream);
 /* Flat float64 vector helpers: out[0] = sum a*b (deterministic, device
  * scalar); out = alpha x + beta y (out may be y). */
 size_t jrx_dot_workspace_bytes(void);

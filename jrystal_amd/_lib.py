@@ -77,6 +77,19 @@ SIGNATURES = {
   "jrx_plan_precondition": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_band_norms": [_P, _P, _P, _SZ, _P],
   "jrx_plan_orthonormal_params": [_P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_subspace_matrix": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_rotate": [_P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_density_stats": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_features": [_P, _P, _I64, _P, _P, _P, _SZ, _P],
+  "jrx_plan_mlp_apply": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_mlp_num_params": [],
+  "jrx_mlp_forward": [_P, _P, _P, _I64, _P, _P],
+  "jrx_mlp_backward_workspace_bytes": [_I64],
+  "jrx_mlp_backward": [_P, _P, _P, _P, _I64, _P, _I, _P, _SZ, _P],
+  "jrx_mlp_cosine_loss": [_P, _P, _P, _P, _I64, _I64, _P, _P, _P],
+  "jrx_mlp_mse_workspace_bytes": [_I64],
+  "jrx_mlp_mse_loss": [_P, _P, _I64, _P, _P, _P, _SZ, _P],
+  "jrx_mlp_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I64, _P],
   "jrx_dot_workspace_bytes": [],
   "jrx_vec_dot": [_P, _P, _I64, _P, _P, _SZ, _P],
   "jrx_vec_axpby": [_D, _P, _D, _P, _P, _I64, _P],
@@ -96,7 +109,10 @@ _RESTYPES = {"jrx_last_error": ctypes.c_char_p,
              "jrx_kinetic_workspace_bytes": ctypes.c_size_t,
              "jrx_reduce_workspace_bytes": ctypes.c_size_t,
              "jrx_ewald_workspace_bytes": ctypes.c_size_t,
-             "jrx_dot_workspace_bytes": ctypes.c_size_t}
+             "jrx_dot_workspace_bytes": ctypes.c_size_t,
+             "jrx_mlp_num_params": ctypes.c_size_t,
+             "jrx_mlp_backward_workspace_bytes": ctypes.c_size_t,
+             "jrx_mlp_mse_workspace_bytes": ctypes.c_size_t}
 
 
 def lib():

@@ -11,6 +11,7 @@
 
 #include "energy_kernels.h"
 #include "fft_kernels.h"
+#include "mlp_core.h"
 #include "qr_kernels.h"
 
 namespace jrx {
@@ -61,6 +62,9 @@ static int cached_fftlen(int n, FftLen* L) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Blocks of the density-statistics reduction.
+constexpr int kStatBlocks = 256;
+
 }  // namespace jrx
 
 using namespace jrx;
@@ -89,6 +93,7 @@ struct jrx_plan {
   std::vector<hipEvent_t> events;
   std::vector<std::pair<int, int>> records;  // (class, event pair index)
   // workspace offsets (bytes)
+  size_t o_stats;
   size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_tb, o_S, o_R1, o_R2, o_Ri, o_A,
       o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, total;
 };
@@ -312,6 +317,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_eh = take((size_t)p->rec_parts * sizeof(double));
   p->o_ee = take((size_t)p->rec_parts * sizeof(double));
   p->o_exc = take((size_t)p->rec_parts * sizeof(double));
+  p->o_stats = take((size_t)(2 * kStatBlocks + 8) * sizeof(double));
   p->total = off;
   return JRX_OK;
 }
@@ -391,6 +397,17 @@ int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info) {
 //   x = |G+k|^2 / tb[state]  (plane-wave over band kinetic energy),
 //   K(x) = (27 + 18x + 12x^2 + 8x^3) / (27 + 18x + 12x^2 + 8x^3 + 16x^4),
 // optionally times a per-coefficient factor (learned preconditioner).
+// |G+k|^2 of mask point g at k-point k.
+__device__ inline double coef_gk2(const short* __restrict__ g_m,
+                                  const double* __restrict__ kpts,
+                                  const CellParams& cp, int g, int k) {
+  const int mx = g_m[3 * g], my = g_m[3 * g + 1], mz = g_m[3 * g + 2];
+  const double gx = mx * cp.B[0] + my * cp.B[3] + mz * cp.B[6] + kpts[3 * k];
+  const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7] + kpts[3 * k + 1];
+  const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8] + kpts[3 * k + 2];
+  return gx * gx + gy * gy + gz * gz;
+}
+
 __global__ void tpa_kernel(const double* __restrict__ gr,
                            const double* __restrict__ gi,
                            const double* __restrict__ tb,
@@ -408,12 +425,8 @@ __global__ void tpa_kernel(const double* __restrict__ gr,
   const int64_t sk = rest / ng;
   const int k = (int)(sk % nk);
   const int64_t st = sk * nb + b;
-  const int mx = g_m[3 * g], my = g_m[3 * g + 1], mz = g_m[3 * g + 2];
-  const double gx = mx * cp.B[0] + my * cp.B[3] + mz * cp.B[6] + kpts[3 * k];
-  const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7] + kpts[3 * k + 1];
-  const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8] + kpts[3 * k + 2];
   const double t = tb[st];
-  const double x = (gx * gx + gy * gy + gz * gz) / (t > 1e-300 ? t : 1e-300);
+  const double x = coef_gk2(g_m, kpts, cp, g, k) / (t > 1e-300 ? t : 1e-300);
   const double p = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
   double kx = p / (p + 16.0 * x * x * x * x);
   if (factor) kx *= factor[i];
@@ -488,6 +501,234 @@ extern "C" int jrx_plan_orthonormal_params(const jrx_plan* p, double* out_re,
                      dim3((unsigned)ceil_div(p->ng, 64), (unsigned)p->P),
                      dim3(256), 0, (hipStream_t)stream, q, p->ng, p->nb,
                      out_re, out_im);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+// out[p][g][b] = sum_c Q[p][c][g] V[p][c][b]: one 32-wide G tile of one
+// (s,k) block per workgroup; the Q tile and V[p] are staged in LDS.
+constexpr int kRotG = 32;
+__global__ void __launch_bounds__(256)
+q_rotate_kernel(const double2* __restrict__ q, const double2* __restrict__ V,
+                int ng, int nb, double* __restrict__ outr,
+                double* __restrict__ outi) {
+  __shared__ double2 tile[kMaxBands][kRotG + 1];
+  __shared__ double2 vs[kMaxBands * kMaxBands];
+  const int64_t pk = blockIdx.y;
+  const int g0 = blockIdx.x * kRotG;
+  const double2* src = q + pk * (int64_t)nb * ng;
+  for (int i = threadIdx.x; i < nb * kRotG; i += 256) {
+    const int c = i / kRotG, gl = i % kRotG, g = g0 + gl;
+    tile[c][gl] = g < ng ? src[(int64_t)c * ng + g] : make_double2(0.0, 0.0);
+  }
+  const double2* vp = V + pk * nb * nb;
+  for (int i = threadIdx.x; i < nb * nb; i += 256) vs[i] = vp[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb * kRotG; i += 256) {
+    const int gl = i / nb, b = i % nb, g = g0 + gl;
+    if (g >= ng) continue;
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < nb; ++c) {
+      const double2 a = tile[c][gl], v = vs[c * nb + b];
+      re += a.x * v.x - a.y * v.y;
+      im += a.x * v.y + a.y * v.x;
+    }
+    const int64_t o = (pk * ng + g) * nb + b;
+    outr[o] = re;
+    outi[o] = im;
+  }
+}
+
+extern "C" int jrx_plan_subspace_matrix(const jrx_plan* p, double* out,
+                                        const void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && out && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  JRX_CHECK_HIP(hipMemcpyAsync(
+      out, static_cast<const char*>(workspace) + p->o_S,
+      (size_t)p->P * p->nb * p->nb * sizeof(double2),
+      hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return JRX_OK;
+}
+
+extern "C" int jrx_plan_rotate(const jrx_plan* p, const double* v,
+                               double* out_re, double* out_im,
+                               const void* workspace, size_t workspace_bytes,
+                               void* stream) {
+  JRX_REQUIRE(p && v && out_re && out_im && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  const double2* q = reinterpret_cast<const double2*>(
+      static_cast<const char*>(workspace) + p->o_q);
+  hipLaunchKernelGGL(q_rotate_kernel,
+                     dim3((unsigned)ceil_div(p->ng, kRotG), (unsigned)p->P),
+                     dim3(256), 0, (hipStream_t)stream, q,
+                     reinterpret_cast<const double2*>(v), p->ng, p->nb,
+                     out_re, out_im);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+// Partial sums of rho_tot and rho_tot^2 over contiguous chunks of the grid
+// (rho_tot = sum over spins), then mean and variance in one block.
+__global__ void __launch_bounds__(256)
+rho_stats_part_kernel(const double* __restrict__ rho, int ns, int64_t N,
+                      int64_t chunk, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < N ? lo + chunk : N;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
+    double r = 0.0;
+    for (int sp = 0; sp < ns; ++sp) r += rho[sp * N + i];
+    s1 += r;
+    s2 += r * r;
+  }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+rho_stats_final_kernel(const double* __restrict__ part, int nblk, int64_t N,
+                       double* __restrict__ out) {
+  __shared__ double red[256];
+  double s1 = 0.0, s2 = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) {
+    s1 += part[2 * b];
+    s2 += part[2 * b + 1];
+  }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  if (threadIdx.x == 0) {
+    const double mean = s1 / (double)N;
+    const double var = s2 / (double)N - mean * mean;
+    out[0] = mean;
+    out[1] = var > 0.0 ? var : 0.0;
+  }
+}
+
+// Raw network features of the coefficients idx[i] (flat [s][k][g][b]
+// index); an index outside the parameter vector gets zero features.
+__global__ void coef_features_kernel(const int64_t* __restrict__ idx,
+                                     int64_t n, int64_t total,
+                                     const double* __restrict__ tb,
+                                     const short* __restrict__ g_m,
+                                     const double* __restrict__ kpts,
+                                     CellParams cp, int ng, int nk, int nb,
+                                     const double* __restrict__ stats,
+                                     float* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c = idx[i];
+  float f[kMlpIn];
+  if (c >= 0 && c < total) {
+    const int b = (int)(c % nb);
+    const int64_t rest = c / nb;
+    const int g = (int)(rest % ng);
+    const int64_t sk = rest / ng;
+    const int k = (int)(sk % nk);
+    mlp_features(coef_gk2(g_m, kpts, cp, g, k), tb[sk * nb + b], stats[0],
+                 stats[1], f);
+  } else {
+    for (int j = 0; j < kMlpIn; ++j) f[j] = 0.f;
+  }
+  for (int j = 0; j < kMlpIn; ++j) x[i * kMlpIn + j] = f[j];
+}
+
+// Network inference on every coefficient: factor[i] = exp(y0) and, when
+// w is given, w[i] *= exp(y1) (warm start), y clamped before exp.
+__global__ void __launch_bounds__(256)
+coef_mlp_kernel(const float* __restrict__ params,
+                const float* __restrict__ norm, int64_t n,
+                const double* __restrict__ tb, const short* __restrict__ g_m,
+                const double* __restrict__ kpts, CellParams cp, int ng,
+                int nk, int nb, const double* __restrict__ stats,
+                double* __restrict__ factor, double* __restrict__ wr,
+                double* __restrict__ wi) {
+  __shared__ float w[kMlpParams];
+  mlp_stage(params, w);
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)(i % nb);
+  const int64_t rest = i / nb;
+  const int g = (int)(rest % ng);
+  const int64_t sk = rest / ng;
+  const int k = (int)(sk % nk);
+  float f[kMlpIn];
+  mlp_features(coef_gk2(g_m, kpts, cp, g, k), tb[sk * nb + b], stats[0],
+               stats[1], f);
+  mlp_normalise(norm, f);
+  float y[kMlpOut];
+  mlp_eval(w, f, y);
+  if (factor)
+    factor[i] = exp((double)fminf(fmaxf(y[0], kLogFactorMin), kLogFactorMax));
+  if (wr) {
+    const double a =
+        exp((double)fminf(fmaxf(y[1], kLogAmpMin), kLogAmpMax));
+    wr[i] *= a;
+    wi[i] *= a;
+  }
+}
+
+extern "C" int jrx_plan_density_stats(const jrx_plan* p, double* out,
+                                      void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  JRX_REQUIRE(p && out && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  char* ws = static_cast<char*>(workspace);
+  const double* rho = reinterpret_cast<const double*>(ws + p->o_rho);
+  double* part = reinterpret_cast<double*>(ws + p->o_stats);
+  const int64_t chunk = (p->N + kStatBlocks - 1) / kStatBlocks;
+  const int nblk = (int)((p->N + chunk - 1) / chunk);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(rho_stats_part_kernel, dim3(nblk), dim3(256), 0, s, rho,
+                     p->ns, p->N, chunk, part);
+  JRX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(rho_stats_final_kernel, dim3(1), dim3(256), 0, s, part,
+                     nblk, p->N, out);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+extern "C" int jrx_plan_features(const jrx_plan* p, const int64_t* idx,
+                                 int64_t n, const double* stats, float* x,
+                                 const void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && idx && stats && x && workspace && n >= 0,
+              "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  if (n == 0) return JRX_OK;
+  const double* tb = reinterpret_cast<const double*>(
+      static_cast<const char*>(workspace) + p->o_tb);
+  hipLaunchKernelGGL(coef_features_kernel, dim3(ceil_div(n, 256)), dim3(256),
+                     0, (hipStream_t)stream, idx, n, p->S * p->ng, tb,
+                     p->d_g_m, p->d_kpts, p->cp, p->ng, p->nk, p->nb, stats,
+                     x);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+extern "C" int jrx_plan_mlp_apply(const jrx_plan* p, const float* params,
+                                  const float* norm, const double* stats,
+                                  double* factor, double* w_re, double* w_im,
+                                  const void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && params && norm && stats && workspace, "null argument");
+  JRX_REQUIRE((w_re == nullptr) == (w_im == nullptr),
+              "w_re and w_im must both be given or both be NULL");
+  JRX_REQUIRE(factor || w_re, "nothing to compute");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  const int64_t n = p->S * p->ng;
+  const double* tb = reinterpret_cast<const double*>(
+      static_cast<const char*>(workspace) + p->o_tb);
+  hipLaunchKernelGGL(coef_mlp_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, params, norm, n, tb, p->d_g_m,
+                     p->d_kpts, p->cp, p->ng, p->nk, p->nb, stats, factor,
+                     w_re, w_im);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

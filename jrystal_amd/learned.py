@@ -1,0 +1,391 @@
+"""Learned diagonal preconditioner and warm start.
+
+A small FP32 network (8 features -> 64 -> 64 -> 2, tanh; HIP kernels in
+csrc/mlp.hip and csrc/plan.hip) maps, for every plane-wave coefficient
+(s,k,G,b), the features
+
+  log1p(|G+k|^2), log1p(|G+k|^2 / tb_b), log tb_b,
+  log rho_mean, log1p(rho_var / rho_mean^2)
+
+(tb_b = twice the band's kinetic energy, rho_mean / rho_var = statistics
+of the current total density) to
+
+  output 0: log of a multiplier of the Teter-Payne-Allan preconditioner,
+  output 1: log amplitude of the warm-start coefficients.
+
+Training data are trajectories of the plain method itself (preconditioned
+CG with the TPA preconditioner and Rayleigh-Ritz gauge resets):
+
+* preconditioner head -- at snapshots W_t of a trajectory that converges
+  to Q*, the direction to the minimum is the Grassmann direction
+  D_t = (I - Q_t Q_t^H) Q* (Q_t^H Q*)^{-1}, mapped to W by R_t.  The head
+  is trained to maximise the cosine between exp(y0) * (P_TPA g_t) and D_t
+  over sampled occupied coefficients (loss 1 - cos^2 per snapshot);
+* warm-start head -- regression of 1/2 log(w(k,G) ng / n_occ) with
+  w = sum_{b occupied} |Q*_{kGb}|^2, the weight of plane wave G+k in the
+  converged occupied subspace, from the features of the random start.
+
+The minimiser (minimize.Minimizer) falls back to the plain TPA
+preconditioner whenever a line search fails to decrease the energy.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .objective import EnergyPlan
+
+NFEAT = 8
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                             "data", "precond_mlp.npz")
+
+
+def num_params() -> int:
+  return int(_lib.lib().jrx_mlp_num_params())
+
+
+class PrecondNet:
+  """Network parameters (flat FP32) and feature normalisation on a device."""
+
+  def __init__(self, params, norm, device=None):
+    self.device = torch.device(device if device is not None else "cuda")
+    self.params = torch.as_tensor(np.asarray(params, np.float32)).to(
+      self.device).contiguous()
+    self.norm = torch.as_tensor(np.asarray(norm, np.float32)).to(
+      self.device).contiguous()
+    if self.params.numel() != num_params() or self.norm.numel() != 2 * NFEAT:
+      raise ValueError("network parameter / normalisation size mismatch")
+
+  @classmethod
+  def initial(cls, seed=0, device=None):
+    """Glorot-uniform hidden layers, a small output layer (outputs start
+    near 0: factor 1, amplitude 1), identity normalisation."""
+    rng = np.random.default_rng(seed)
+    h, nin = 64, NFEAT
+    parts = []
+    lim1 = np.sqrt(6.0 / (nin + h))
+    parts.append(rng.uniform(-lim1, lim1, (h, nin)))
+    parts.append(np.zeros(h))
+    lim2 = np.sqrt(6.0 / (2 * h))
+    parts.append(rng.uniform(-lim2, lim2, (h, h)))
+    parts.append(np.zeros(h))
+    parts.append(rng.uniform(-0.01, 0.01, (2, h)))
+    parts.append(np.zeros(2))
+    p = np.concatenate([x.ravel() for x in parts]).astype(np.float32)
+    norm = np.concatenate([np.zeros(nin), np.ones(nin)]).astype(np.float32)
+    return cls(p, norm, device)
+
+  @classmethod
+  def load(cls, path=DEFAULT_MODEL, device=None):
+    with np.load(path, allow_pickle=False) as z:
+      return cls(z["params"], z["norm"], device)
+
+  def save(self, path, **meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=self.params.cpu().numpy(),
+             norm=self.norm.cpu().numpy(),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+  def forward(self, x):
+    """y (n, 2) for raw features x (n, 8) (device float32)."""
+    x = x.to(device=self.device, dtype=torch.float32).contiguous()
+    y = torch.empty((x.shape[0], 2), dtype=torch.float32, device=self.device)
+    _lib.check(_lib.lib().jrx_mlp_forward(
+      _lib.ptr(self.params), _lib.ptr(self.norm), _lib.ptr(x), x.shape[0],
+      _lib.ptr(y), _lib.stream_ptr(self.device)))
+    return y
+
+
+# --------------------------------------------------------- plan helpers ----
+def density_stats(plan: EnergyPlan, out=None):
+  """(mean, variance) of the total density of the last evaluation."""
+  if out is None:
+    out = torch.empty(2, dtype=torch.float64, device=plan.device)
+  _lib.check(_lib.lib().jrx_plan_density_stats(
+    plan._plan, _lib.ptr(out), plan._ws_ptr, plan.workspace_bytes,
+    _lib.stream_ptr(plan.device)))
+  return out
+
+
+def coefficient_features(plan: EnergyPlan, idx, stats):
+  """Raw features (n, 8) float32 of flat coefficient indices idx."""
+  idx = torch.as_tensor(idx, dtype=torch.int64).to(plan.device).contiguous()
+  x = torch.empty((idx.numel(), NFEAT), dtype=torch.float32,
+                  device=plan.device)
+  _lib.check(_lib.lib().jrx_plan_features(
+    plan._plan, _lib.ptr(idx), idx.numel(), _lib.ptr(stats), _lib.ptr(x),
+    plan._ws_ptr, plan.workspace_bytes, _lib.stream_ptr(plan.device)))
+  return x
+
+
+def apply(plan: EnergyPlan, net: PrecondNet, stats, factor=None, w_re=None,
+          w_im=None):
+  """Per-coefficient inference: factor = exp(y0), w *= exp(y1)."""
+  _lib.check(_lib.lib().jrx_plan_mlp_apply(
+    plan._plan, _lib.ptr(net.params), _lib.ptr(net.norm), _lib.ptr(stats),
+    _lib.ptr(factor), _lib.ptr(w_re), _lib.ptr(w_im), plan._ws_ptr,
+    plan.workspace_bytes, _lib.stream_ptr(plan.device)))
+
+
+def warm_start(plan: EnergyPlan, net: PrecondNet, params):
+  """Initial guess: the seeded start W0 reshaped coefficient-wise by the
+  network's amplitude, W = exp(y1(features of W0)) * W0."""
+  shape = (2,) + tuple(plan.param_shape)
+  W = torch.empty(shape, dtype=torch.float64, device=plan.device)
+  W[0].copy_(torch.as_tensor(params["w_re"]))
+  W[1].copy_(torch.as_tensor(params["w_im"]))
+  plan.evaluate(W[0], W[1], grad=False)
+  apply(plan, net, density_stats(plan), w_re=W[0], w_im=W[1])
+  return {"w_re": W[0], "w_im": W[1]}
+
+
+class LearnedFactor:
+  """factor_fn of minimize.Minimizer: the network's preconditioner factor,
+  recomputed at every gauge reset (from the evaluation that reset made) and
+  held fixed between resets, so that CG sees one preconditioner per
+  cycle."""
+
+  def __init__(self, plan: EnergyPlan, net: PrecondNet):
+    self.plan, self.net = plan, net
+    self.factor = torch.empty(plan.param_shape, dtype=torch.float64,
+                              device=plan.device)
+    self.stats = torch.empty(2, dtype=torch.float64, device=plan.device)
+    self._stamp = None
+
+  def __call__(self, minimizer):
+    if minimizer.restarts != self._stamp:
+      density_stats(self.plan, self.stats)
+      apply(self.plan, self.net, self.stats, factor=self.factor)
+      self._stamp = minimizer.restarts
+    return self.factor
+
+
+# ------------------------------------------------------------- training ----
+@dataclass
+class Dataset:
+  """Samples of the two heads (host arrays)."""
+  x_pc: np.ndarray    # (nsnap * snap, 8) features, preconditioner head
+  a: np.ndarray       # Re(conj(D) P_TPA g)
+  b: np.ndarray       # |P_TPA g|^2
+  t2: np.ndarray      # (nsnap,) sum |D|^2 over the snapshot's samples
+  snap: int
+  x_ws: np.ndarray    # (m, 8) features of the random start, warm-start head
+  y_ws: np.ndarray    # (m,) target log amplitude
+
+  @staticmethod
+  def concat(parts: List["Dataset"]) -> "Dataset":
+    snap = parts[0].snap
+    assert all(p.snap == snap for p in parts)
+    return Dataset(np.concatenate([p.x_pc for p in parts]),
+                   np.concatenate([p.a for p in parts]),
+                   np.concatenate([p.b for p in parts]),
+                   np.concatenate([p.t2 for p in parts]), snap,
+                   np.concatenate([p.x_ws for p in parts]),
+                   np.concatenate([p.y_ws for p in parts]))
+
+  def save(self, path):
+    np.savez(path, x_pc=self.x_pc, a=self.a, b=self.b, t2=self.t2,
+             snap=self.snap, x_ws=self.x_ws, y_ws=self.y_ws)
+
+  @staticmethod
+  def load(path):
+    with np.load(path, allow_pickle=False) as z:
+      return Dataset(z["x_pc"], z["a"], z["b"], z["t2"], int(z["snap"]),
+                     z["x_ws"], z["y_ws"])
+
+
+def _occupied_block(occ):
+  """Number of leading occupied bands, required equal for every (s,k)."""
+  pos = occ > 0
+  nocc = pos.sum(-1)
+  n = int(nocc.flat[0])
+  if not np.all(nocc == n) or not np.all(pos[..., :n]):
+    raise NotImplementedError("trajectory sampling needs the same leading "
+                              "block of occupied bands at every (s,k)")
+  return n
+
+
+def _host_complex(re, im):
+  return re.cpu().numpy() + 1j * im.cpu().numpy()
+
+
+def grassmann_direction(q_t, q_star):
+  """D = (I - q_t q_t^H) q_star (q_t^H q_star)^{-1} per leading index: the
+  tangent at span(q_t) with span(q_t + D) = span(q_star) (q_* (..., G, n)
+  with orthonormal columns)."""
+  m = np.conj(np.swapaxes(q_t, -1, -2)) @ q_star
+  return (q_star - q_t @ m) @ np.linalg.inv(m)
+
+
+def collect_trajectory(wl, seed: int, snap_every: int = 8,
+                       samples: int = 512, ws_samples: int = 4096,
+                       max_iter: int = 400, max_snaps: int = 24,
+                       rng: Optional[np.random.Generator] = None,
+                       device=None) -> Dataset:
+  """One plain-method trajectory of workload wl from wl.params(seed), turned
+  into training samples of both heads."""
+  from .minimize import Minimizer
+  rng = rng if rng is not None else np.random.default_rng(seed)
+  plan = EnergyPlan.from_workload(wl, device=device)
+  ns, nk, ng, nb = plan.param_shape
+  nocc = _occupied_block(plan.occupation)
+
+  def sample_idx(n):
+    s = rng.integers(0, ns, n)
+    k = rng.integers(0, nk, n)
+    g = rng.integers(0, ng, n)
+    b = rng.integers(0, nocc, n)
+    return ((s * nk + k) * ng + g) * nb + b
+
+  p0 = wl.params(seed)
+  plan.evaluate(p0["w_re"], p0["w_im"], grad=False)
+  idx_ws = sample_idx(ws_samples)
+  x_ws = coefficient_features(plan, idx_ws, density_stats(plan)).cpu().numpy()
+
+  snaps = []
+
+  def cb(m, it, e):
+    if it % snap_every or len(snaps) >= max_snaps:
+      return
+    qr, qi = plan.orthonormal_params()
+    q = _host_complex(qr, qi)[..., :nocc]
+    w = _host_complex(m.W[0], m.W[1])[..., :nocc]
+    idx = sample_idx(samples)
+    h = (m.h[0].reshape(-1)[torch.as_tensor(idx, device=plan.device)].cpu()
+         .numpy() + 1j * m.h[1].reshape(-1)[torch.as_tensor(
+           idx, device=plan.device)].cpu().numpy())
+    x = coefficient_features(plan, idx, density_stats(plan)).cpu().numpy()
+    snaps.append((q, w, idx, h, x))
+
+  m = Minimizer(plan, p0)
+  m.run(max_iter=max_iter, callback=cb)
+  qr, qi = plan.orthonormal_params()
+  qs = _host_complex(qr, qi)[..., :nocc]
+
+  xs, as_, bs, ts = [], [], [], []
+  for q, w, idx, h, x in snaps:
+    ov = np.linalg.svd(np.conj(np.swapaxes(q, -1, -2)) @ qs, compute_uv=False)
+    if ov.min() < 0.2:  # far from the minimum: D is ill-defined
+      continue
+    d = grassmann_direction(q, qs)                   # (s,k,g,nocc)
+    r = np.conj(np.swapaxes(q, -1, -2)) @ w          # R_oo
+    dw = (d @ r).reshape(ns, nk, ng, nocc)
+    b_ = idx % nb
+    sk_g = idx // nb
+    t = dw.reshape(-1, nocc)[sk_g, b_]
+    xs.append(x)
+    as_.append(np.real(np.conj(t) * h))
+    bs.append(np.abs(h)**2)
+    ts.append(np.sum(np.abs(t)**2))
+  wgt = np.sum(np.abs(qs)**2, axis=-1)                # (s,k,g)
+  wnorm = wgt * ng / nocc
+  sk_g = idx_ws // nb
+  y_ws = 0.5 * np.log(np.maximum(wnorm.reshape(-1)[sk_g], 1e-26))
+  f32 = np.float32
+  if not ts:
+    xs, as_, bs, ts = [np.zeros((0, NFEAT))], [np.zeros(0)], [np.zeros(0)], []
+  return Dataset(np.concatenate(xs).astype(f32),
+                 np.concatenate(as_).astype(f32),
+                 np.concatenate(bs).astype(f32),
+                 np.asarray(ts, f32), samples, x_ws.astype(f32),
+                 np.clip(y_ws, -30.0, 8.0).astype(f32))
+
+
+def feature_norm(ds: Dataset):
+  x = np.concatenate([ds.x_pc, ds.x_ws]).astype(np.float64)
+  mu = x.mean(0)
+  sd = x.std(0)
+  inv = np.where(sd > 1e-6, 1.0 / np.maximum(sd, 1e-6), 1.0)
+  return np.concatenate([mu, inv]).astype(np.float32)
+
+
+class Trainer:
+  """Full-batch training of both heads with the HIP kernels:
+  loss = mean_snapshots(1 - cos^2) + ws_weight * mse(warm start)."""
+
+  def __init__(self, net: PrecondNet, ds: Dataset, lr=1e-3, b1=0.9,
+               b2=0.999, ws_weight=0.1):
+    dev = net.device
+    self.net, self.lr, self.b1, self.b2 = net, lr, b1, b2
+    self.ws_weight = ws_weight
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)  # noqa
+    self.x_pc, self.a, self.b, self.t2 = f(ds.x_pc), f(ds.a), f(ds.b), f(ds.t2)
+    self.snap, self.nsnap = ds.snap, len(ds.t2)
+    self.x_ws, self.y_ws = f(ds.x_ws), f(ds.y_ws)
+    n = max(self.x_pc.shape[0], self.x_ws.shape[0])
+    L = _lib.lib()
+    self.ws = torch.empty(max(L.jrx_mlp_backward_workspace_bytes(n),
+                              L.jrx_mlp_mse_workspace_bytes(n)) // 4 + 64,
+                          dtype=torch.float32, device=dev)
+    self.grad = torch.zeros(num_params(), dtype=torch.float32, device=dev)
+    self.m = torch.zeros_like(self.grad)
+    self.v = torch.zeros_like(self.grad)
+    self.loss_pc = torch.empty(max(self.nsnap, 1), dtype=torch.float32,
+                               device=dev)
+    self.loss_ws = torch.empty(1, dtype=torch.float32, device=dev)
+    self.dy_pc = torch.empty((self.x_pc.shape[0], 2), dtype=torch.float32,
+                             device=dev)
+    self.dy_ws = torch.empty((self.x_ws.shape[0], 2), dtype=torch.float32,
+                             device=dev)
+    self.count = 0
+
+  def loss_and_grad(self):
+    """Fills self.grad; returns (cosine loss, warm-start mse) tensors."""
+    L, net, s = _lib.lib(), self.net, _lib.stream_ptr(self.net.device)
+    wsb = self.ws.numel() * 4
+    acc = 0
+    if self.nsnap:
+      y = net.forward(self.x_pc)
+      _lib.check(L.jrx_mlp_cosine_loss(
+        _lib.ptr(y), _lib.ptr(self.a), _lib.ptr(self.b), _lib.ptr(self.t2),
+        self.snap, self.nsnap, _lib.ptr(self.dy_pc), _lib.ptr(self.loss_pc),
+        s))
+      _lib.check(L.jrx_mlp_backward(
+        _lib.ptr(net.params), _lib.ptr(net.norm), _lib.ptr(self.x_pc),
+        _lib.ptr(self.dy_pc), self.x_pc.shape[0], _lib.ptr(self.grad), 0,
+        _lib.ptr(self.ws), wsb, s))
+      acc = 1
+    if self.x_ws.shape[0] and self.ws_weight > 0:
+      y = net.forward(self.x_ws)
+      _lib.check(L.jrx_mlp_mse_loss(
+        _lib.ptr(y), _lib.ptr(self.y_ws), self.x_ws.shape[0],
+        _lib.ptr(self.dy_ws), _lib.ptr(self.loss_ws), _lib.ptr(self.ws), wsb,
+        s))
+      self.dy_ws.mul_(self.ws_weight)
+      _lib.check(L.jrx_mlp_backward(
+        _lib.ptr(net.params), _lib.ptr(net.norm), _lib.ptr(self.x_ws),
+        _lib.ptr(self.dy_ws), self.x_ws.shape[0], _lib.ptr(self.grad), acc,
+        _lib.ptr(self.ws), wsb, s))
+    return self.loss_pc[:self.nsnap].mean(), self.loss_ws[0]
+
+  def step(self):
+    lpc, lws = self.loss_and_grad()
+    self.count += 1
+    _lib.check(_lib.lib().jrx_mlp_adam(
+      _lib.ptr(self.net.params), _lib.ptr(self.grad), _lib.ptr(self.m),
+      _lib.ptr(self.v), self.grad.numel(), ctypes.c_double(self.lr),
+      ctypes.c_double(self.b1), ctypes.c_double(self.b2),
+      ctypes.c_double(1e-8), self.count, _lib.stream_ptr(self.net.device)))
+    return lpc, lws
+
+
+def train(ds: Dataset, epochs=400, lr=2e-3, seed=0, ws_weight=0.1,
+          device=None, log_every=0):
+  """Train a fresh network on ds; returns (net, history of (cos, mse))."""
+  net = PrecondNet.initial(seed, device)
+  net.norm.copy_(torch.as_tensor(feature_norm(ds)))
+  tr = Trainer(net, ds, lr=lr, ws_weight=ws_weight)
+  hist = []
+  for ep in range(epochs):
+    lpc, lws = tr.step()
+    if log_every and (ep % log_every == 0 or ep == epochs - 1):
+      hist.append((ep, float(lpc), float(lws)))
+      print(f"epoch {ep}: cos-loss {hist[-1][1]:.5f} ws-mse {hist[-1][2]:.4f}",
+            flush=True)
+  lpc, lws = tr.loss_and_grad()
+  hist.append((epochs, float(lpc), float(lws)))
+  return net, hist

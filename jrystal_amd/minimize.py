@@ -5,21 +5,29 @@ The reference minimises with optax Adam on the raw parameters
 classic plane-wave alternative: preconditioned conjugate gradients on W
 (Teter, Payne & Allan, PRB 40, 12255 (1989)) with
 
-* search direction d = -P g + beta d_prev (Polak-Ribiere+, restarted every
-  ``restart`` iterations or when d is not a descent direction);
+* search direction d = -P g + beta d_prev (Polak-Ribiere+), restarted every
+  ``restart`` iterations or when d is not a descent direction;
+* a gauge reset at every restart: W <- Q V, the QR factor of W (E is
+  invariant under W -> W T, T upper triangular) rotated by the
+  Rayleigh-Ritz eigenvectors V within each block of equally occupied
+  bands (E is invariant there too), so every column is an approximate
+  eigenstate with R = I;
 * P = diag(K(|G+k|^2 / tb_band)) (the TPA kinetic preconditioner), times an
-  optional per-coefficient factor -- the hook of the learned preconditioner;
-* a line search that fits a parabola through E(0), E'(0) and one
-  energy-only trial E(l_t), then evaluates energy and gradient at the
-  parabola's minimum;
-* acceptance only when the energy decreases; otherwise the trial point is
-  taken if it decreased, else the step shrinks and the direction restarts
-  from -P g (the fall-back of the learned preconditioner uses the same
-  rule, with the plain preconditioner).
+  optional per-coefficient factor -- the hook of the learned
+  preconditioner, held fixed within a restart cycle;
+* a line search with the initial step lam_prev s0_prev / s0 (Nocedal &
+  Wright 3.5) that fits a parabola through E(0), E'(0) and one energy-only
+  trial E(lt), then evaluates energy and gradient at its minimum (or, with
+  line_search="secant", one evaluation with the gradient and a secant
+  correction);
+* acceptance only when the energy does not increase; otherwise up to
+  ``backtracks`` parabolic backtracking steps, and if those fail too, a
+  fall-back to the plain preconditioner for two restart cycles from a
+  steepest-descent restart.
 
 Every vector operation is a HIP kernel (jrx_vec_dot, jrx_vec_axpby,
-jrx_plan_precondition); the only host traffic is a handful of scalars per
-iteration.
+jrx_plan_precondition, jrx_plan_rotate); the host sees a handful of
+scalars per iteration and the small subspace matrices at restarts.
 """
 import ctypes
 import time
@@ -42,12 +50,19 @@ class MinimizeResult:
   history: List[tuple] = field(default_factory=list)  # (seconds, energy)
   rejected: int = 0
   fallbacks: int = 0
+  status: str = ""
 
 
 class Minimizer:
 
   def __init__(self, plan: EnergyPlan, params, factor_fn: Optional[Callable]
-               = None, restart: int = 60, step0: float = 0.5):
+               = None, restart: int = 60, step0: float = 0.5,
+               line_search: str = "parabola", ritz: bool = True):
+    if line_search not in ("secant", "parabola"):
+      raise ValueError(f"unknown line search {line_search!r}")
+    self.line_search = line_search
+    self.ritz = ritz
+    self.slope_tol = 1e-15
     self.plan = plan
     dev = plan.device
     shape = (2,) + tuple(plan.param_shape)
@@ -65,8 +80,14 @@ class Minimizer:
     self._dws = torch.empty(ws + 256, dtype=torch.uint8, device=dev)
     self._dws_ptr = (self._dws.data_ptr() + 255) // 256 * 256
     self.factor_fn = factor_fn
+    self.restarts = 0       # gauge resets so far
+    self._plain_cycles = 0  # restart cycles left on the plain preconditioner
     self.restart = restart
     self.step = step0
+    self.backtracks = 6
+    self.flat_window = 12
+    self.flat_tol = 1e-14
+    self._s0_prev = None
     self.evals = 0
 
   # -- kernels -------------------------------------------------------------
@@ -77,9 +98,9 @@ class Minimizer:
     self.plan.evaluate(W[0], W[1], grad=g is not None, out=out)
     self.evals += 1
 
-  def _precondition(self, g, h, use_factor=True):
+  def _precondition(self, g, h):
     factor = None
-    if use_factor and self.factor_fn is not None:
+    if self.factor_fn is not None and self._plain_cycles == 0:
       factor = self.factor_fn(self)
     self.plan.precondition(g[0], g[1], factor, h[0], h[1])
 
@@ -93,26 +114,97 @@ class Minimizer:
       ctypes.c_double(alpha), x.data_ptr(), ctypes.c_double(beta),
       y.data_ptr(), out.data_ptr(), self.n, _lib.stream_ptr()))
 
-  def _restart(self, use_factor=True):
-    """W <- Q of the last evaluation (same energy, gauge R reset to I),
+  def _ritz_rotation(self):
+    """Unitary V (s,k,nb,nb): on each run of consecutive bands with equal
+    occupation f > 0, the eigenvectors of the subspace Hamiltonian
+    (Q^H Qbar) / 2f in ascending order; identity elsewhere.  Needs the
+    last evaluation to carry a gradient."""
+    S = self.plan.subspace_matrix().cpu().numpy()
+    occ = self.plan.occupation
+    ns, nk, nb = occ.shape
+    V = np.broadcast_to(np.eye(nb, dtype=complex), S.shape).copy()
+    rows = occ.reshape(-1, nb)
+    Sf = S.reshape(-1, nb, nb)
+    Vf = V.reshape(-1, nb, nb)
+    for pattern in np.unique(rows, axis=0):
+      sel = np.nonzero(np.all(rows == pattern, axis=1))[0]
+      b = 0
+      while b < nb:
+        e = b
+        while e + 1 < nb and pattern[e + 1] == pattern[b]:
+          e += 1
+        if pattern[b] > 0 and e > b:
+          h = Sf[sel, b:e + 1, b:e + 1] / (2.0 * pattern[b])
+          h = 0.5 * (h + np.conj(np.swapaxes(h, -1, -2)))
+          Vf[sel, b:e + 1, b:e + 1] = np.linalg.eigh(h)[1]
+        b = e + 1
+    return torch.from_numpy(V)
+
+  def _restart(self):
+    """Gauge reset W <- Q V of the last evaluation (same energy; R = I and,
+    with ``ritz``, Rayleigh-Ritz bands within each occupation block), the
     gradient at the new W, and the preconditioned steepest-descent
-    direction.  Returns (E, g.Pg)."""
-    self.plan.orthonormal_params(self.W[0], self.W[1])
+    direction.  The last evaluation must carry a gradient.  Returns
+    (E, g.Pg)."""
+    if self.ritz:
+      self.plan.rotate(self._ritz_rotation(), self.W[0], self.W[1])
+    else:
+      self.plan.orthonormal_params(self.W[0], self.W[1])
     self._eval(self.W, self.e, self.g)
-    self._precondition(self.g, self.h, use_factor)
+    self.restarts += 1
+    self._precondition(self.g, self.h)
     self._axpby(-1.0, self.h, 0.0, self.h, self.d)
     self._dot(self.g, self.h, 0)
     return float(self.e[4]), float(self.dotv[0])
+
+  # -- line searches along d from W (energy E0, slope s0 = g.d < 0) --------
+  def _search_parabola(self, E0, s0, lt):
+    """Energy-only trial at lt, then energy and gradient at the minimum of
+    the parabola through E(0), E'(0) and E(lt).  Two evaluations."""
+    self._axpby(1.0, self.W, lt, self.d, self.W1)
+    self._eval(self.W1, self.e1)
+    Et = float(self.e1[4])
+    c = (Et - E0 - s0 * lt) / (lt * lt)
+    lam = -s0 / (2.0 * c) if c > 0 else 4.0 * lt
+    lam = min(max(lam, 0.05 * lt), 8.0 * lt)
+    self._axpby(1.0, self.W, lam, self.d, self.W1)
+    self._eval(self.W1, self.e1, self.g1)
+    self._next_step = lam
+    return lam, float(self.e1[4]), Et
+
+  def _search_secant(self, E0, s0, lt):
+    """Energy and gradient at the predicted step lt; the slope there gives
+    the secant minimum lam* = lt s0 / (s0 - s1).  The point is kept when
+    lam* is within 30% of lt (the usual case once the step has adapted),
+    else one more evaluation at lam*.  lam* predicts the next step."""
+    self._axpby(1.0, self.W, lt, self.d, self.W1)
+    self._eval(self.W1, self.e1, self.g1)
+    Et = float(self.e1[4])
+    self._dot(self.g1, self.d, 2)
+    s1 = float(self.dotv[2])
+    if np.isfinite(Et) and s0 - s1 > 0:
+      lam = lt * s0 / (s0 - s1)
+    else:
+      c = (Et - E0 - s0 * lt) / (lt * lt) if np.isfinite(Et) else 1.0
+      lam = -s0 / (2.0 * c) if c > 0 else 4.0 * lt
+    lam = min(max(lam, 0.05 * lt), 8.0 * lt)
+    self._next_step = lam
+    if abs(lam - lt) <= 0.3 * lt and np.isfinite(Et) and Et <= E0:
+      return lt, Et, Et
+    self._axpby(1.0, self.W, lam, self.d, self.W1)
+    self._eval(self.W1, self.e1, self.g1)
+    return lam, float(self.e1[4]), Et
 
   # -- driver --------------------------------------------------------------
   def run(self, max_iter: int = 200, e_stop: Optional[float] = None,
           time_limit: Optional[float] = None, callback=None):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    self._eval(self.W, self.e)  # Q of the starting point
+    self._eval(self.W, self.e, self.g)  # Q of the starting point
     E0, gh = self._restart()
     hist = [(time.perf_counter() - t0, E0)]
     rejected = fallbacks = stalled = 0
+    status = "max_iter"
     since_restart = 0
     it = 0
     for it in range(1, max_iter + 1):
@@ -122,44 +214,63 @@ class Minimizer:
         self._axpby(-1.0, self.h, 0.0, self.h, self.d)
         s0 = -gh
         since_restart = 0
+      if -s0 <= self.slope_tol * max(1.0, abs(E0)):
+        status = "converged"  # slope at the rounding level of E
+        break
+      # initial trial step: keep the first-order decrease lam * s0 of the
+      # last accepted step (Nocedal & Wright, Numerical Optimization, 3.5)
       lt = self.step
-      # energy-only trial, then the minimum of the parabola through
-      # E(0), E'(0) = s0 and E(lt)
-      self._axpby(1.0, self.W, lt, self.d, self.W1)
-      self._eval(self.W1, self.e1)
-      Et = float(self.e1[4])
-      c = (Et - E0 - s0 * lt) / (lt * lt)
-      lam = -s0 / (2.0 * c) if c > 0 else 4.0 * lt
-      lam = min(max(lam, 0.05 * lt), 8.0 * lt)
-      self._axpby(1.0, self.W, lam, self.d, self.W1)
-      self._eval(self.W1, self.e1, self.g1)
-      E1 = float(self.e1[4])
+      if self._s0_prev is not None:
+        lt = self.step * min(max(self._s0_prev / s0, 1e-4), 1e4)
+      if not (np.isfinite(lt) and lt > 1e-150):
+        status = "stalled"  # step collapsed (rounding-level slopes)
+        break
+      if self.line_search == "parabola":
+        lam, E1, Et = self._search_parabola(E0, s0, lt)
+      else:
+        lam, E1, Et = self._search_secant(E0, s0, lt)
       if not np.isfinite(E1) or E1 > E0:
         rejected += 1
         if np.isfinite(Et) and Et < E0:
           lam = lt
+        for _ in range(self.backtracks):
+          if np.isfinite(E1) and E1 <= E0:
+            break
+          if not (np.isfinite(Et) and Et < E0 and lam == lt):
+            # minimum of the parabola through E(0), E'(0), E(lam), kept in
+            # [0.1, 0.5] lam
+            c = (E1 - E0 - s0 * lam) / (lam * lam) if np.isfinite(E1) else 0
+            lam_p = -s0 / (2 * c) if c > 0 else 0.1 * lam
+            lam = min(max(lam_p, 0.1 * lam), 0.5 * lam)
+          Et = np.nan
           self._axpby(1.0, self.W, lam, self.d, self.W1)
           self._eval(self.W1, self.e1, self.g1)
           E1 = float(self.e1[4])
+        self._next_step = lam
         if not np.isfinite(E1) or E1 > E0:
-          # fall back: plain preconditioner, steepest descent, shorter step
+          # fall back: plain preconditioner for this restart cycle and the
+          # next, steepest descent from a gauge reset
           fallbacks += 1
           stalled += 1
-          self.step = lt * 0.2
-          self._eval(self.W, self.e)
-          E0, gh = self._restart(use_factor=False)
+          self.step = lam
+          self._s0_prev = None
+          self._plain_cycles = 2
+          self._eval(self.W, self.e, self.g)
+          E0, gh = self._restart()
           since_restart = 0
           hist.append((time.perf_counter() - t0, E0))
-          if stalled >= 4:
+          if stalled >= 3:
+            status = "stalled"
             break
           continue
       stalled = 0
-      self.step = lam
+      self.step = self._next_step
+      self._s0_prev = s0
       self.W, self.W1 = self.W1, self.W
       self.e, self.e1 = self.e1, self.e
       since_restart += 1
       if since_restart >= self.restart:
-        self.g, self.g1 = self.g1, self.g
+        self._plain_cycles = max(0, self._plain_cycles - 1)
         E1, gh = self._restart()
         since_restart = 0
       else:
@@ -177,13 +288,20 @@ class Minimizer:
       if callback is not None:
         callback(self, it, E0)
       if e_stop is not None and E0 <= e_stop:
+        status = "e_stop"
+        break
+      if (len(hist) > self.flat_window and hist[-self.flat_window - 1][1] - E0
+          <= self.flat_tol * max(1.0, abs(E0))):
+        status = "converged"  # no decrease above rounding for a while
         break
       if time_limit is not None and hist[-1][0] > time_limit:
+        status = "time_limit"
         break
     torch.cuda.synchronize()
     return MinimizeResult(energy=E0, iterations=it, evaluations=self.evals,
                           seconds=time.perf_counter() - t0, history=hist,
-                          rejected=rejected, fallbacks=fallbacks)
+                          rejected=rejected, fallbacks=fallbacks,
+                          status=status)
 
   def params(self):
     return {"w_re": self.W[0], "w_im": self.W[1]}

@@ -88,6 +88,7 @@ class EnergyPlan:
     self.chunk_states = info.chunk_states
     self.keep_planes = bool(info.keep_planes)
     self.ns, self.nk, self.nb = occ.shape
+    self.occupation = occ.copy()
     self.grid_sizes = mask.shape
     self.param_shape = (self.ns, self.nk, self.num_g, self.nb)
     self.workspace_bytes = info.workspace_bytes
@@ -200,6 +201,36 @@ class EnergyPlan:
       _lib.check(self._lib.jrx_plan_orthonormal_params(
         self._plan, _lib.ptr(out_re), _lib.ptr(out_im), self._ws_ptr,
         self.workspace_bytes, _lib.stream_ptr(self.device)))
+    return out_re, out_im
+
+  def subspace_matrix(self, out=None):
+    """Q^H Qbar of the last evaluation with a gradient, (ns,nk,nb,nb)
+    complex128 on the device; for bands of equal occupation f > 0 it is
+    2 f q_i^H H q_j."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb, self.nb),
+                        dtype=torch.complex128, device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_subspace_matrix(
+        self._plan, _lib.ptr(out), self._ws_ptr, self.workspace_bytes,
+        _lib.stream_ptr(self.device)))
+    return out
+
+  def rotate(self, v, out_re=None, out_im=None):
+    """Q V of the last evaluation in the (s,k,g,b) parameter layout;
+    v is (ns,nk,nb,nb) complex128."""
+    v = v.to(device=self.device, dtype=torch.complex128).contiguous()
+    if tuple(v.shape) != (self.ns, self.nk, self.nb, self.nb):
+      raise ValueError(f"rotation has shape {tuple(v.shape)}")
+    shape = self.param_shape
+    if out_re is None:
+      out_re = torch.empty(shape, dtype=torch.float64, device=self.device)
+    if out_im is None:
+      out_im = torch.empty(shape, dtype=torch.float64, device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_rotate(
+        self._plan, _lib.ptr(v), _lib.ptr(out_re), _lib.ptr(out_im),
+        self._ws_ptr, self.workspace_bytes, _lib.stream_ptr(self.device)))
     return out_re, out_im
 
   def set_timing(self, enable: bool):

@@ -1,0 +1,110 @@
+"""Train the learned preconditioner / warm-start network on trajectories of
+the plain method over random cells, with the project's HIP kernels.
+
+  python -m jrystal_amd.train_precond --out model.npz [--trajectories N]
+      [--seconds S] [--epochs E] [--data dataset.npz]
+
+Random cells (numpy.random.default_rng(seed)):
+  * fcc2  -- 2-atom FCC cells, a = 10.26 (1 + U(-0.05, 0.05)) bohr, a random
+    symmetric strain of amplitude 0.1, atoms displaced by N(0, 0.15) bohr,
+    Z in {3, 4}; 32^3 grid, 2x2x2 k-mesh, 8 bands, cutoff 24 Ha;
+  * cube8 -- 8-atom diamond cubes, a = 10.26 bohr, strain amplitude 0.1,
+    displacements N(0, 0.15) bohr, Z = 4; 64^3 grid, Gamma, 32 bands,
+    cutoff 48 Ha.
+Cells whose minimum-image atom separation is below 2.5 bohr are redrawn.
+Trajectory seeds start at 1000, away from the strained held-out cell
+(strain seed 4).
+"""
+import argparse
+import time
+
+import numpy as np
+
+from . import learned
+from . import workloads as W
+from .crystal import Crystal
+
+
+def _min_separation(cell, pos):
+  best = np.inf
+  shifts = np.array([[i, j, k] for i in (-1, 0, 1) for j in (-1, 0, 1)
+                     for k in (-1, 0, 1)], float) @ cell
+  for a in range(len(pos)):
+    for b in range(len(pos)):
+      d = pos[b] - pos[a] + shifts
+      n = np.linalg.norm(d, axis=1)
+      if a == b:
+        n = n[n > 1e-9]
+      best = min(best, n.min())
+  return best
+
+
+def random_cell(seed: int, kind: str):
+  rng = np.random.default_rng(seed)
+  while True:
+    strain = W.strain_matrix(int(rng.integers(1 << 30)), 0.1)
+    if kind == "fcc2":
+      z = float(rng.choice([3.0, 4.0]))
+      a = W.SI_A * (1 + rng.uniform(-0.05, 0.05))
+      base = W.fcc_si2(z)
+      cell = base.cell_vectors * (a / W.SI_A) @ (np.eye(3) + strain)
+      frac = np.array([[0, 0, 0], [0.25, 0.25, 0.25]])
+      grid, kgrid, nb, ecut = (32,) * 3, (2, 2, 2), 8, 24.0
+    else:
+      z = 4.0
+      base = W.diamond_cube(z)
+      cell = base.cell_vectors @ (np.eye(3) + strain)
+      frac = base.positions @ np.linalg.inv(base.cell_vectors)
+      grid, kgrid, nb, ecut = (64,) * 3, (1, 1, 1), 32, 48.0
+    pos = frac @ cell + rng.normal(0, 0.15, (len(frac), 3))
+    if _min_separation(cell, pos) < 2.5:
+      continue
+    cr = Crystal(charges=np.full(len(pos), z), positions=pos,
+                 cell_vectors=cell, spin=0, symbols=["X"] * len(pos))
+    return W.build(f"{kind}-{seed}", cr, grid, kgrid, nb, ecut)
+
+
+def generate(n: int, seconds: float, seed0: int = 1000, cube_every: int = 4,
+             log=print):
+  parts, t0 = [], time.perf_counter()
+  for i in range(n):
+    if time.perf_counter() - t0 > seconds:
+      break
+    kind = "cube8" if i % cube_every == cube_every - 1 else "fcc2"
+    wl = random_cell(seed0 + i, kind)
+    parts.append(learned.collect_trajectory(wl, seed=seed0 + i))
+    if log and (i % 20 == 0):
+      log(f"trajectory {i} ({kind}): {len(parts[-1].t2)} snapshots, "
+          f"{time.perf_counter() - t0:.1f} s")
+  return learned.Dataset.concat(parts), len(parts)
+
+
+def main(argv=None):
+  ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+  ap.add_argument("--out", required=True)
+  ap.add_argument("--data", default=None, help="save / reuse the dataset")
+  ap.add_argument("--trajectories", type=int, default=2000)
+  ap.add_argument("--seconds", type=float, default=600.0)
+  ap.add_argument("--epochs", type=int, default=600)
+  ap.add_argument("--lr", type=float, default=2e-3)
+  args = ap.parse_args(argv)
+  import os
+  if args.data and os.path.exists(args.data):
+    ds = learned.Dataset.load(args.data)
+    ntraj = -1
+  else:
+    ds, ntraj = generate(args.trajectories, args.seconds)
+    if args.data:
+      ds.save(args.data)
+  print(f"dataset: {ntraj} trajectories, {len(ds.t2)} snapshots, "
+        f"{ds.x_ws.shape[0]} warm-start samples", flush=True)
+  net, hist = learned.train(ds, epochs=args.epochs, lr=args.lr,
+                            log_every=max(1, args.epochs // 10))
+  net.save(args.out, trajectories=ntraj, snapshots=len(ds.t2),
+           history=np.asarray(hist))
+  print(f"saved {args.out}", flush=True)
+  return 0
+
+
+if __name__ == "__main__":
+  raise SystemExit(main())

@@ -7,6 +7,8 @@ numpy.random.default_rng(seed): w_re first, then w_im, standard normal.
   2x2x2 Monkhorst-Pack k-mesh (8 k), 8 bands, spherical cutoff 24 Ha.
 * ``si8_cube`` -- conventional diamond Si cube a = 10.26 bohr (8 atoms),
   64^3 grid, 4x4x4 k-mesh (64 k), 32 bands, spherical cutoff 48 Ha.
+* ``si8_strained`` -- si8_cube under a random symmetric strain (held-out
+  case of the preconditioner comparison).
 * ``si64``     -- 2x2x2 supercell of the cube (a = 20.52 bohr, 64 atoms),
   Gamma only, 128^3 grid, 128 bands, cutoff 48 Ha.
 """
@@ -89,6 +91,21 @@ def si2_fcc(z=14.0):
 def si8_cube(z=14.0, k=4, strain=None):
   return build(f"si8-cube-64^3-{k**3}k-32b", diamond_cube(z, strain=strain),
                (64,) * 3, (k, k, k), 32, 48.0)
+
+
+def strain_matrix(seed=4, amplitude=0.1):
+  """Symmetric strain with entries ~ U(-amplitude, amplitude):
+  e = rng.uniform(-a, a, (3, 3)), strain = (e + e^T) / 2."""
+  e = np.random.default_rng(seed).uniform(-amplitude, amplitude, (3, 3))
+  return (e + e.T) / 2
+
+
+def si8_strained(z=4.0, k=4, seed=4, amplitude=0.1):
+  """The 8-atom cube under a random symmetric lattice strain, cell
+  a (I + strain), same grid, k-mesh, bands and cutoff as si8_cube."""
+  wl = si8_cube(z, k, strain=strain_matrix(seed, amplitude))
+  wl.name = f"si8-strained{seed}-64^3-{k**3}k-32b"
+  return wl
 
 
 def si64(z=4.0):

@@ -177,3 +177,17 @@ def test_register_fft64_matches_generic(monkeypatch):
   assert np.max(np.abs(a["density"] - b["density"])) < 1e-13
   assert rel_max(a["dw_re"], b["dw_re"]) < 1e-12
   assert rel_max(a["dw_im"], b["dw_im"]) < 1e-12
+
+
+def test_strained_gamma_only_si8():
+  """Gamma-only (one k-point), sheared cell, 32 bands: the shape of the
+  strained 8-atom comparison case at nk = 1."""
+  from jrystal_amd import workloads as W
+  _case(W.si8_strained(4.0, k=1), seed=2)
+  wl = W.si8_strained(4.0, k=1)
+  plan = _plan(wl.crystal, wl.mask, wl.kpts, wl.occupation)
+  params = seeded_params(wl.shape, 2)
+  full = plan.evaluate(params["w_re"], params["w_im"], grad=True)
+  eo = plan.evaluate(params["w_re"], params["w_im"], grad=False)
+  torch.cuda.synchronize()
+  assert (full["energies"] - eo["energies"]).abs().max().item() < 1e-10
