@@ -7,9 +7,22 @@ LDA-x energies, potential and the full adjoint back to dE/dw) of the
 64^3 grid, 4x4x4 Monkhorst-Pack k-mesh (64 k), 32 bands, spherical cutoff
 48 Ha (G_n = 17071), parameters N(0,1) from seed 0, FP64 throughout.
 
+A second measurement, "convergence", times direct minimisation to
+|E - E*| < 1e-6 Ha from the seeded start on the held-out strained 8-atom
+cell (Z = 4, 64 k-points): preconditioned CG with the plain Teter-Payne-
+Allan preconditioner against the learned preconditioner (with and without
+the learned warm start).
+
 Prints ONE JSON line (rank 0).  Flags: --gpus N (only 1 is supported: the
 path runs on one GPU), --steps K, --warmup W, --no-cpu-baseline,
---cpu-seconds S (target length of the CPU baseline sample).
+--cpu-seconds S (target length of the CPU baseline sample),
+--no-convergence, --conv-seconds S (time cap of each convergence run),
+--no-batch, --batch-steps K (the uneven-structure batch section),
+--traffic-probe (the workload of the rocprofv3 counter runs).
+
+A third section, "batch", times Adam steps (energy + gradient + update) of
+16 uneven structures run as one batch: eager, over several streams, and as
+a replayed hipGraph.
 """
 import argparse
 import json
@@ -100,6 +113,159 @@ def cpu_baseline(wl, params, target_s, nthreads):
                      f"{nthreads}")}
 
 
+def time_to(history, e_star, tol):
+  """First (seconds, index) of a (seconds, energy) history within tol."""
+  for i, (t, e) in enumerate(history):
+    if e - e_star < tol:
+      return t, i
+  return None, None
+
+
+def convergence(time_cap, tol=1e-6):
+  """Wall time to |E - E*| < tol from the seeded start, plain (TPA) against
+  learned preconditioning, on the held-out strained 8-atom cell (Z = 4,
+  64^3 grid, 4x4x4 k-mesh, 32 bands, 48 Ha): E* = the lowest energy any of
+  the runs reaches (each run continues to convergence or time_cap)."""
+  import torch
+  from jrystal_amd import learned, workloads
+  from jrystal_amd.minimize import Minimizer
+  from jrystal_amd.objective import EnergyPlan
+  if not os.path.exists(learned.DEFAULT_MODEL):
+    print("bench: no trained preconditioner model; convergence skipped",
+          file=sys.stderr)
+    return None, None
+  wl = workloads.si8_strained(4.0, k=4)
+  plan = EnergyPlan.from_workload(wl)
+  net = learned.PrecondNet.load(learned.DEFAULT_MODEL)
+  runs = {}
+  e_best = None
+  for mode in ("learned", "learned_warm", "plain"):
+    print(f"bench: convergence run {mode}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p0 = wl.params(0)
+    if mode == "learned_warm":
+      p0 = learned.warm_start(plan, net, p0)
+    fn = None if mode == "plain" else learned.LearnedFactor(plan, net)
+    m = Minimizer(plan, p0, factor_fn=fn)
+    t_setup = time.perf_counter() - t0
+    stop = None if e_best is None else e_best + 0.1 * tol
+    r = m.run(max_iter=100000, time_limit=time_cap, e_stop=stop)
+    r.history = [(t + t_setup, e) for t, e in r.history]
+    runs[mode] = r
+    e_best = r.energy if e_best is None else min(e_best, r.energy)
+  e_star = min(r.energy for r in runs.values())
+  out = {"workload": wl.name, "tol_Ha": tol, "E_star_Ha": e_star,
+         "time_cap_s": time_cap}
+  for mode, r in runs.items():
+    t, i = time_to(r.history, e_star, tol)
+    out[mode] = {"seconds": t, "iterations": i,
+                 "evaluations_total": r.evaluations,
+                 "final_E_Ha": r.energy, "status": r.status,
+                 "fallbacks": r.fallbacks}
+  tp, tl = out["plain"]["seconds"], out["learned"]["seconds"]
+  out["speedup_learned_vs_plain"] = (tp / tl) if tp and tl else None
+  tw = out["learned_warm"]["seconds"]
+  out["speedup_learned_warm_vs_plain"] = (tp / tw) if tp and tw else None
+  return out, (wl, runs["plain"])
+
+
+def batch_bench(steps, warmup, streams=4):
+  """Adam steps of the 16 uneven structures of workloads.batch_cells(3)
+  (grids 36^3-80^3, 1-64 k-points, 4-47 bands): eager on one stream, eager
+  over `streams` streams, and the hipGraph-captured step replayed."""
+  import numpy as np
+  import torch
+  from jrystal_amd import workloads
+  from jrystal_amd.batch import BatchStep
+  wls = workloads.batch_cells(seed=3, n=16)
+  out = {"workload": "batch_cells(seed=3): 16 structures",
+         "parameters": int(sum(np.prod(w.shape) for w in wls)),
+         "states": int(sum(w.shape[0] * w.shape[1] * w.shape[3]
+                           for w in wls))}
+  for mode, ns, graph in (("eager_1_stream", 0, False),
+                          (f"eager_{streams}_streams", streams, False),
+                          (f"graph_{streams}_streams", streams, True)):
+    print(f"bench: batch {mode}", file=sys.stderr, flush=True)
+    bs = BatchStep(wls, streams=ns)
+    if graph:
+      bs.capture()
+    run = bs.replay if graph else bs.step
+    for _ in range(warmup):
+      run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+      run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    e = bs.energies()
+    if not np.all(np.isfinite(e)):
+      raise RuntimeError(f"batch {mode}: non-finite energies")
+    out[mode] = {"ms_per_step": dt * 1e3,
+                 "structure_evals_per_s": len(wls) / dt}
+    bs.close()
+    del bs
+    torch.cuda.empty_cache()
+  return out
+
+
+TRAFFIC_PROFILE = os.path.join(
+  os.path.dirname(os.path.abspath(__file__)), "profiles",
+  "energy_grad_si8-cube-64k-32b_pmc-traffic.md")
+KERNEL_NAMES = {"xline_vpsi": "jrx::xline_vpsi64_kernel",
+                "plane_ifft": "jrx::plane_ifft64_kernel",
+                "xline_density": "jrx::xline_density64_kernel",
+                "plane_fft_gather": "jrx::plane_fft_gather64_kernel"}
+
+
+def measured_traffic(cls):
+  """HBM bytes per launch of a kernel class from the committed rocprofv3
+  FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
+  None."""
+  name = KERNEL_NAMES.get(cls)
+  if name is None or not os.path.exists(TRAFFIC_PROFILE):
+    return None
+  text = open(TRAFFIC_PROFILE).read()
+  start = text.find("```json")
+  if start < 0:
+    return None
+  end = text.find("```", start + 7)
+  table = json.loads(text[start + 7:end])
+  return table.get(name)
+
+
+def traffic_probe():
+  """Workload of the rocprofv3 counter runs: a calibration kernel of known
+  traffic (jrx_vec_axpby on 2^27 float64: reads 2 GiB, writes 1 GiB), then
+  two energy+gradient evaluations of the benchmark workload."""
+  import ctypes
+  import torch
+  from jrystal_amd import _lib, workloads
+  from jrystal_amd.objective import EnergyPlan
+  n = 1 << 27
+  x = torch.ones(n, dtype=torch.float64, device="cuda")
+  y = torch.ones(n, dtype=torch.float64, device="cuda")
+  z = torch.empty(n, dtype=torch.float64, device="cuda")
+  for _ in range(2):
+    _lib.check(_lib.lib().jrx_vec_axpby(
+      ctypes.c_double(1.0), x.data_ptr(), ctypes.c_double(2.0), y.data_ptr(),
+      z.data_ptr(), n, _lib.stream_ptr()))
+  torch.cuda.synchronize()
+  del x, y, z
+  wl = workloads.si8_cube(14.0, k=4)
+  p = wl.params(0)
+  plan = EnergyPlan.from_workload(wl)
+  wr = torch.tensor(p["w_re"], device="cuda")
+  wi = torch.tensor(p["w_im"], device="cuda")
+  out = {}
+  for _ in range(2):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  torch.cuda.synchronize()
+  print(f"traffic probe done: E = {float(out['energies'][4]):.10f}",
+        flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -108,7 +274,16 @@ def main():
   ap.add_argument("--no-cpu-baseline", action="store_true")
   ap.add_argument("--cpu-seconds", type=float, default=12.0)
   ap.add_argument("--chunk-states", type=int, default=0)
+  ap.add_argument("--no-convergence", action="store_true")
+  ap.add_argument("--conv-seconds", type=float, default=30.0)
+  ap.add_argument("--no-batch", action="store_true")
+  ap.add_argument("--batch-steps", type=int, default=5)
+  ap.add_argument("--traffic-probe", action="store_true",
+                  help="run the rocprofv3 counter workload and exit")
   args = ap.parse_args()
+  if args.traffic_probe:
+    traffic_probe()
+    return
   if args.gpus != 1:
     print(f"bench.py: --gpus {args.gpus} is not supported; this path runs "
           "on one GPU (k-point sharding across GPUs is a design note only).",
@@ -168,15 +343,50 @@ def main():
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(dom),
                 "avg_launch_ms": per_launch_ms,
                 "algorithmic_bytes_per_launch": per_launch_bytes}
   breakdown = {k: round(v[0] / nprof, 4) for k, v in timing.items()}
+  config = {"workload": wl.name, "atoms": int(wl.crystal.num_atom),
+            "Z": 14, "grid": list(wl.grid_sizes), "k_mesh": list(wl.k_grid),
+            "num_kpts": int(plan.nk), "num_bands": int(plan.nb),
+            "num_g": int(plan.num_g), "cutoff_Ha": wl.cutoff_energy,
+            "chunk_states": int(plan.chunk_states),
+            "keep_planes": bool(plan.keep_planes),
+            "energy_total_Ha": energies[4]}
+
+  plan.close()
+  del plan, out, wr, wi
+  torch.cuda.empty_cache()
+
+  batch = None
+  if not args.no_batch:
+    batch = batch_bench(args.batch_steps, 1)
+
+  conv = None
+  if not args.no_convergence:
+    conv, conv_ctx = convergence(args.conv_seconds)
 
   cpu = None
   if not args.no_cpu_baseline:
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     cpu = cpu_baseline(wl, params, args.cpu_seconds, nthreads)
+    if conv is not None:
+      # CPU seconds per evaluation of the convergence cell, times the
+      # evaluations the plain method needed on the GPU
+      cwl, plain = conv_ctx
+      c2 = cpu_baseline(cwl, cwl.params(0), args.cpu_seconds / 2, nthreads)
+      i = conv["plain"]["iterations"]
+      if i is not None and plain.history:
+        frac = i / max(1, len(plain.history) - 1)
+        evals = frac * plain.evaluations
+        conv["cpu_estimate_plain_s"] = evals / c2["value"]
+        conv["cpu_estimate_note"] = (
+          "estimate: evaluations the plain method used on the GPU up to the "
+          f"tolerance ({evals:.0f}) / CPU evaluations per second of the "
+          f"oracle restatement on this cell ({c2['value']:.4g}/s, "
+          f"{c2['sample']})")
 
   line = {
     "metric": METRIC, "value": value,
@@ -184,16 +394,12 @@ def main():
     "steps": args.steps, "warmup": args.warmup,
     "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
     "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-    "config": {"workload": wl.name, "atoms": int(wl.crystal.num_atom),
-               "Z": 14, "grid": list(wl.grid_sizes), "k_mesh": list(wl.k_grid),
-               "num_kpts": int(plan.nk), "num_bands": int(plan.nb),
-               "num_g": int(plan.num_g), "cutoff_Ha": wl.cutoff_energy,
-               "chunk_states": int(plan.chunk_states),
-               "keep_planes": bool(plan.keep_planes),
-               "energy_total_Ha": energies[4]},
+    "config": config,
     "roofline": roofline,
     "kernel_ms_per_eval": breakdown,
     "cpu_baseline": cpu,
+    "convergence": conv,
+    "batch": batch,
   }
   print(json.dumps(line))
 

@@ -69,6 +69,8 @@ typedef struct {
   const double* occupation;/* host [ns*nk*nb]                               */
   int xc;                  /* JRX_XC_LDA_X                                  */
   int chunk_states;        /* states per FFT chunk, 0 = automatic           */
+  int fft_variant;         /* 0 = automatic, 1 = runtime-radix kernels only */
+  int xline_blocks;        /* target workgroups of an x-line pass, 0 = 4096 */
 } jrx_problem;
 
 int jrx_plan_create(jrx_plan** plan, const jrx_problem* problem);
@@ -83,6 +85,8 @@ typedef struct {
   size_t workspace_bytes;
 } jrx_plan_info;
 int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info);
+/* Number of float64 values in w_re (= in w_im): ns*nk*ng*nb. */
+int jrx_plan_param_count(const jrx_plan* plan, int64_t* count);
 
 /* One evaluation.  w_re, w_im: [ns][nk][ng][nb] (pw.param_init layout,
  * pw.py:29-91).  energies: device [5] = kinetic, hartree, external, xc,
@@ -139,10 +143,12 @@ int jrx_plan_rotate(const jrx_plan* plan, const double* v, double* out_re,
  * W2[64][64], b2[64], W3[2][64], b3[2]) and norm[16] = (mu[8], 1/sd[8])
  * applied to the raw features first.  Raw features of coefficient
  * (s,k,g,b): log1p(|G+k|^2), log1p(|G+k|^2 / tb), log(tb),
- * log(rho_mean), log1p(rho_var / rho_mean^2), 0, 0, 0 with tb the band
- * norm and rho_mean / rho_var the statistics of the total density of the
- * last evaluation.  Output 0 is the log preconditioner factor (clamped to
- * [-8, 8]), output 1 the log warm-start amplitude (clamped to [-30, 8]).
+ * log(rho_mean), log1p(rho_var / rho_mean^2), log1p(gap), 0, 0 with tb the
+ * band norm, rho_mean / rho_var the statistics of the total density of the
+ * last evaluation and gap[s][k][b] a caller-given per-band value (the
+ * band's energy below the highest occupied band; NULL = 0).  Output 0 is
+ * the log preconditioner factor (clamped to [-8, 8]), output 1 the log
+ * warm-start amplitude (clamped to [-30, 8]).
  * ---------------------------------------------------------------------- */
 /* out[0] = mean, out[1] = variance of the total density over the grid. */
 int jrx_plan_density_stats(const jrx_plan* plan, double* out,
@@ -151,13 +157,15 @@ int jrx_plan_density_stats(const jrx_plan* plan, double* out,
 /* x[i][0..7] = raw features of coefficient idx[i] (device int64 flat
  * [s][k][g][b] indices; out-of-range indices give zeros). */
 int jrx_plan_features(const jrx_plan* plan, const int64_t* idx, int64_t n,
-                      const double* stats, float* x, const void* workspace,
-                      size_t workspace_bytes, void* stream);
+                      const double* stats, const double* gap, float* x,
+                      const void* workspace, size_t workspace_bytes,
+                      void* stream);
 /* Inference on every coefficient: factor = exp(y0) (if factor is not
  * NULL), w *= exp(y1) (if w_re / w_im are not NULL). */
 int jrx_plan_mlp_apply(const jrx_plan* plan, const float* params,
                        const float* norm, const double* stats,
-                       double* factor, double* w_re, double* w_im,
+                       const double* gap, double* factor, double* w_re,
+                       double* w_im,
                        const void* workspace, size_t workspace_bytes,
                        void* stream);
 /* Plain network kernels (training).  x [n][8] raw features, y [n][2]. */
@@ -294,6 +302,43 @@ int jrx_plan_set_timing(jrx_plan* plan, int enable);
 /* Synchronises on the recorded events, writes the summed milliseconds and
  * launch counts per class since the last read, and resets them. */
 int jrx_plan_read_timing(jrx_plan* plan, double* ms, int* launches);
+
+/* ------------------------------------------------------------------------
+ * Batched optimiser step over plans of different shapes (no single
+ * counterpart in the reference: its driver steps one structure,
+ * calc_ground_state_energy_all_electrons.py:175-220).  One step = for
+ * every member, jrx_energy_grad into its gradient buffers, then one grouped
+ * optax.adam update (opt_utils.py:153-168 rule) of all members' w_re and
+ * w_im with the step count t kept on the device (incremented by the step,
+ * so a captured step can be replayed).  Members run on `streams` auxiliary
+ * streams (0 = all on the caller's stream) between fork / join events.
+ * jrx_batch_capture records one step into a hipGraph; jrx_batch_replay
+ * launches it.  All buffers are device buffers owned by the caller.
+ * ---------------------------------------------------------------------- */
+typedef struct jrx_batch jrx_batch;
+typedef struct {
+  jrx_plan* plan;
+  double *w_re, *w_im;     /* parameters, updated in place                  */
+  double *dw_re, *dw_im;   /* gradient buffers                              */
+  double *m_re, *m_im;     /* Adam first moments                            */
+  double *v_re, *v_im;     /* Adam second moments                           */
+  double* energies;        /* [5], written by every step                    */
+  void* workspace;         /* jrx_plan_workspace_bytes, 256-byte aligned    */
+  size_t workspace_bytes;
+} jrx_batch_member;
+typedef struct {
+  double lr, b1, b2, eps, eps_root;
+} jrx_adam_config;
+int jrx_batch_create(jrx_batch** batch, const jrx_batch_member* members,
+                     int n, const jrx_adam_config* adam, int streams);
+int jrx_batch_destroy(jrx_batch* batch);
+int jrx_batch_set_step_count(jrx_batch* batch, int64_t count, void* stream);
+/* Synchronises `stream`. */
+int jrx_batch_get_step_count(const jrx_batch* batch, int64_t* count,
+                             void* stream);
+int jrx_batch_step(jrx_batch* batch, void* stream);
+int jrx_batch_capture(jrx_batch* batch, void* stream);
+int jrx_batch_replay(jrx_batch* batch, void* stream);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,24 @@ class JrxProblem(ctypes.Structure):
     ("cell", ctypes.c_void_p), ("positions", ctypes.c_void_p),
     ("charges", ctypes.c_void_p), ("occupation", ctypes.c_void_p),
     ("xc", ctypes.c_int), ("chunk_states", ctypes.c_int),
+    ("fft_variant", ctypes.c_int), ("xline_blocks", ctypes.c_int),
   ]
+
+
+class JrxBatchMember(ctypes.Structure):
+  _fields_ = [("plan", ctypes.c_void_p),
+              ("w_re", ctypes.c_void_p), ("w_im", ctypes.c_void_p),
+              ("dw_re", ctypes.c_void_p), ("dw_im", ctypes.c_void_p),
+              ("m_re", ctypes.c_void_p), ("m_im", ctypes.c_void_p),
+              ("v_re", ctypes.c_void_p), ("v_im", ctypes.c_void_p),
+              ("energies", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
+              ("workspace_bytes", ctypes.c_size_t)]
+
+
+class JrxAdamConfig(ctypes.Structure):
+  _fields_ = [("lr", ctypes.c_double), ("b1", ctypes.c_double),
+              ("b2", ctypes.c_double), ("eps", ctypes.c_double),
+              ("eps_root", ctypes.c_double)]
 
 
 class JrxPlanInfo(ctypes.Structure):
@@ -53,6 +70,16 @@ SIGNATURES = {
   "jrx_plan_destroy": [_P],
   "jrx_plan_workspace_bytes": [_P, ctypes.POINTER(_SZ)],
   "jrx_plan_get_info": [_P, ctypes.POINTER(JrxPlanInfo)],
+  "jrx_plan_param_count": [_P, ctypes.POINTER(_I64)],
+  "jrx_batch_create": [ctypes.POINTER(ctypes.c_void_p),
+                       ctypes.POINTER(JrxBatchMember), _I,
+                       ctypes.POINTER(JrxAdamConfig), _I],
+  "jrx_batch_destroy": [_P],
+  "jrx_batch_set_step_count": [_P, _I64, _P],
+  "jrx_batch_get_step_count": [_P, ctypes.POINTER(_I64), _P],
+  "jrx_batch_step": [_P, _P],
+  "jrx_batch_capture": [_P, _P],
+  "jrx_batch_replay": [_P, _P],
   "jrx_energy_grad": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
   "jrx_qr_workspace_bytes": [_I64, _I, _I],
   "jrx_qr": [_P, _P, _I64, _I, _I, _P, _P, _P, _SZ, _P],
@@ -80,8 +107,8 @@ SIGNATURES = {
   "jrx_plan_subspace_matrix": [_P, _P, _P, _SZ, _P],
   "jrx_plan_rotate": [_P, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_density_stats": [_P, _P, _P, _SZ, _P],
-  "jrx_plan_features": [_P, _P, _I64, _P, _P, _P, _SZ, _P],
-  "jrx_plan_mlp_apply": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_features": [_P, _P, _I64, _P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_mlp_apply": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
   "jrx_mlp_num_params": [],
   "jrx_mlp_forward": [_P, _P, _P, _I64, _P, _P],
   "jrx_mlp_backward_workspace_bytes": [_I64],

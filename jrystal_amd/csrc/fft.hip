@@ -168,10 +168,11 @@ int xlines_per_block(int nx) {
   return LZ < 1 ? 1 : LZ;
 }
 
-int xline_groups(const BoxGeom& geo, int nstates) {
+int xline_groups(const BoxGeom& geo, int nstates, int target_blocks) {
   const int LZ = xlines_per_block(geo.nx);
   const int tiles = geo.ny * ((geo.nz + LZ - 1) / LZ);
-  int G = (4096 + tiles - 1) / tiles;
+  const int target = target_blocks > 0 ? target_blocks : 4096;
+  int G = (target + tiles - 1) / tiles;
   if (G > kMaxXGroups) G = kMaxXGroups;
   if (G > nstates) G = nstates;
   return G < 1 ? 1 : G;

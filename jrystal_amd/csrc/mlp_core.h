@@ -60,9 +60,12 @@ __device__ inline void mlp_eval(const float* __restrict__ w,
 }
 
 // Raw features of one coefficient: |G+k|^2, the band's tb (twice its
-// kinetic energy) and the density mean / variance of the current iterate.
-__device__ inline void mlp_features(double gk2, double tb, double rho_mean,
-                                    double rho_var, float x[kMlpIn]) {
+// kinetic energy), its energy gap below the highest occupied band of its
+// (s,k) block (Rayleigh quotients; 0 for empty bands) and the density mean /
+// variance of the current iterate.
+__device__ inline void mlp_features(double gk2, double tb, double gap,
+                                    double rho_mean, double rho_var,
+                                    float x[kMlpIn]) {
   const double t = tb > 1e-300 ? tb : 1e-300;
   const double m = rho_mean > 1e-300 ? rho_mean : 1e-300;
   x[0] = (float)log1p(gk2);
@@ -70,7 +73,7 @@ __device__ inline void mlp_features(double gk2, double tb, double rho_mean,
   x[2] = (float)log(t);
   x[3] = (float)log(m);
   x[4] = (float)log1p((rho_var > 0.0 ? rho_var : 0.0) / (m * m));
-  x[5] = 0.f;
+  x[5] = (float)log1p(gap > 0.0 ? gap : 0.0);
   x[6] = 0.f;
   x[7] = 0.f;
 }

@@ -86,6 +86,7 @@ struct jrx_plan {
   bool force_generic = false;  // run the runtime-radix kernels at 64^3
   bool keep_planes = false;    // plane buffer holds all states
   int groups = 1;  // state groups of the x-line kernels
+  int xline_blocks = 0;  // target workgroups of an x-line pass (0 = 4096)
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
   // timing
@@ -281,9 +282,14 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   // JRX_FFT_GENERIC=1 selects the runtime-radix FFT kernels even where the
   // register-resident 64-point ones apply (used to cross-check them).
   const char* gen = getenv("JRX_FFT_GENERIC");
-  p->force_generic = gen && gen[0] == '1';
-  p->groups = xline_groups(p->geo, p->keep_planes ? (int)spin_states
-                                                   : p->chunk);
+  JRX_REQUIRE(d->fft_variant == 0 || d->fft_variant == 1,
+              "fft_variant must be 0 (automatic) or 1 (runtime radix)");
+  JRX_REQUIRE(d->xline_blocks >= 0, "xline_blocks must be >= 0");
+  p->force_generic = (gen && gen[0] == '1') || d->fft_variant == 1;
+  p->xline_blocks = d->xline_blocks > 0 ? d->xline_blocks : 0;
+  p->groups = xline_groups(p->geo,
+                           p->keep_planes ? (int)spin_states : p->chunk,
+                           p->xline_blocks);
   p->gsw = gram_split(p->P, ng);
   p->gsq = p->gsw;
   const int64_t kin_gather = p->S * p->geo.nxa;
@@ -379,6 +385,12 @@ int jrx_plan_destroy(jrx_plan* plan) {
 int jrx_plan_workspace_bytes(const jrx_plan* plan, size_t* bytes) {
   JRX_REQUIRE(plan && bytes, "null argument");
   *bytes = plan->total;
+  return JRX_OK;
+}
+
+int jrx_plan_param_count(const jrx_plan* plan, int64_t* count) {
+  JRX_REQUIRE(plan && count, "null argument");
+  *count = plan->S * plan->ng;
   return JRX_OK;
 }
 
@@ -619,6 +631,7 @@ __global__ void coef_features_kernel(const int64_t* __restrict__ idx,
                                      const double* __restrict__ kpts,
                                      CellParams cp, int ng, int nk, int nb,
                                      const double* __restrict__ stats,
+                                     const double* __restrict__ gap,
                                      float* __restrict__ x) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -630,8 +643,9 @@ __global__ void coef_features_kernel(const int64_t* __restrict__ idx,
     const int g = (int)(rest % ng);
     const int64_t sk = rest / ng;
     const int k = (int)(sk % nk);
-    mlp_features(coef_gk2(g_m, kpts, cp, g, k), tb[sk * nb + b], stats[0],
-                 stats[1], f);
+    const int64_t st = sk * nb + b;
+    mlp_features(coef_gk2(g_m, kpts, cp, g, k), tb[st], gap ? gap[st] : 0.0,
+                 stats[0], stats[1], f);
   } else {
     for (int j = 0; j < kMlpIn; ++j) f[j] = 0.f;
   }
@@ -646,6 +660,7 @@ coef_mlp_kernel(const float* __restrict__ params,
                 const double* __restrict__ tb, const short* __restrict__ g_m,
                 const double* __restrict__ kpts, CellParams cp, int ng,
                 int nk, int nb, const double* __restrict__ stats,
+                const double* __restrict__ gap,
                 double* __restrict__ factor, double* __restrict__ wr,
                 double* __restrict__ wi) {
   __shared__ float w[kMlpParams];
@@ -659,8 +674,9 @@ coef_mlp_kernel(const float* __restrict__ params,
   const int64_t sk = rest / ng;
   const int k = (int)(sk % nk);
   float f[kMlpIn];
-  mlp_features(coef_gk2(g_m, kpts, cp, g, k), tb[sk * nb + b], stats[0],
-               stats[1], f);
+  const int64_t st = sk * nb + b;
+  mlp_features(coef_gk2(g_m, kpts, cp, g, k), tb[st], gap ? gap[st] : 0.0,
+               stats[0], stats[1], f);
   mlp_normalise(norm, f);
   float y[kMlpOut];
   mlp_eval(w, f, y);
@@ -695,7 +711,8 @@ extern "C" int jrx_plan_density_stats(const jrx_plan* p, double* out,
 }
 
 extern "C" int jrx_plan_features(const jrx_plan* p, const int64_t* idx,
-                                 int64_t n, const double* stats, float* x,
+                                 int64_t n, const double* stats,
+                                 const double* gap, float* x,
                                  const void* workspace,
                                  size_t workspace_bytes, void* stream) {
   JRX_REQUIRE(p && idx && stats && x && workspace && n >= 0,
@@ -707,14 +724,15 @@ extern "C" int jrx_plan_features(const jrx_plan* p, const int64_t* idx,
   hipLaunchKernelGGL(coef_features_kernel, dim3(ceil_div(n, 256)), dim3(256),
                      0, (hipStream_t)stream, idx, n, p->S * p->ng, tb,
                      p->d_g_m, p->d_kpts, p->cp, p->ng, p->nk, p->nb, stats,
-                     x);
+                     gap, x);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
 
 extern "C" int jrx_plan_mlp_apply(const jrx_plan* p, const float* params,
                                   const float* norm, const double* stats,
-                                  double* factor, double* w_re, double* w_im,
+                                  const double* gap, double* factor,
+                                  double* w_re, double* w_im,
                                   const void* workspace,
                                   size_t workspace_bytes, void* stream) {
   JRX_REQUIRE(p && params && norm && stats && workspace, "null argument");
@@ -727,8 +745,8 @@ extern "C" int jrx_plan_mlp_apply(const jrx_plan* p, const float* params,
       static_cast<const char*>(workspace) + p->o_tb);
   hipLaunchKernelGGL(coef_mlp_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
                      (hipStream_t)stream, params, norm, n, tb, p->d_g_m,
-                     p->d_kpts, p->cp, p->ng, p->nk, p->nb, stats, factor,
-                     w_re, w_im);
+                     p->d_kpts, p->cp, p->ng, p->nk, p->nb, stats, gap,
+                     factor, w_re, w_im);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -846,14 +864,14 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
         JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst, p->geo,
                                                     p->Lx.tw, s));
         JRX_T(JRX_K_XLINE_DENSITY,
-              launch_xline_density64(Tc, rpart, xline_groups(p->geo, nst),
+              launch_xline_density64(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
                                      p->d_occ, st0, nst, 1.0 / p->vol,
                                      p->geo, p->Lx.tw, s));
       } else {
         JRX_T(JRX_K_PLANE_IFFT,
               launch_plane_ifft(q, Tc, st0, nst, p->geo, p->Ly, p->Lz, s));
         JRX_T(JRX_K_XLINE_DENSITY,
-              launch_xline_density(Tc, rpart, xline_groups(p->geo, nst),
+              launch_xline_density(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
                                    p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
                                    p->Lx, s));
       }
@@ -892,7 +910,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
                                                         p->geo, p->Lx.tw, s));
           JRX_T(JRX_K_XLINE_VPSI,
                 launch_xline_vpsi64(Tc, v + sp * p->N,
-                                    xline_groups(p->geo, nst), nst, p->geo,
+                                    xline_groups(p->geo, nst, p->xline_blocks), nst, p->geo,
                                     p->Lx.tw, s));
           JRX_T(JRX_K_PLANE_FFT_GATHER,
                 launch_plane_fft_gather64(Tc, q, qbar, kin, st0, nst, p->geo,
@@ -902,7 +920,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
             JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, Tc, st0, nst, p->geo,
                                                       p->Ly, p->Lz, s));
           JRX_T(JRX_K_XLINE_VPSI,
-                launch_xline_vpsi(Tc, v + sp * p->N, xline_groups(p->geo, nst),
+                launch_xline_vpsi(Tc, v + sp * p->N, xline_groups(p->geo, nst, p->xline_blocks),
                                   nst, p->geo, p->Lx, s));
           JRX_T(JRX_K_PLANE_FFT_GATHER,
                 launch_plane_fft_gather(Tc, q, qbar, kin, st0, nst, p->geo,

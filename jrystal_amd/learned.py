@@ -5,10 +5,11 @@ csrc/mlp.hip and csrc/plan.hip) maps, for every plane-wave coefficient
 (s,k,G,b), the features
 
   log1p(|G+k|^2), log1p(|G+k|^2 / tb_b), log tb_b,
-  log rho_mean, log1p(rho_var / rho_mean^2)
+  log rho_mean, log1p(rho_var / rho_mean^2), log1p(gap_b)
 
-(tb_b = twice the band's kinetic energy, rho_mean / rho_var = statistics
-of the current total density) to
+(tb_b = twice the band's kinetic energy, gap_b = its Rayleigh-quotient
+energy below the highest occupied band of its k-point, rho_mean / rho_var
+= statistics of the current total density) to
 
   output 0: log of a multiplier of the Teter-Payne-Allan preconditioner,
   output 1: log amplitude of the warm-start coefficients.
@@ -51,8 +52,9 @@ def num_params() -> int:
 class PrecondNet:
   """Network parameters (flat FP32) and feature normalisation on a device."""
 
-  def __init__(self, params, norm, device=None):
+  def __init__(self, params, norm, device=None, use_gap=True):
     self.device = torch.device(device if device is not None else "cuda")
+    self.use_gap = bool(use_gap)  # feed the band-gap feature (slot 5)
     self.params = torch.as_tensor(np.asarray(params, np.float32)).to(
       self.device).contiguous()
     self.norm = torch.as_tensor(np.asarray(norm, np.float32)).to(
@@ -61,7 +63,7 @@ class PrecondNet:
       raise ValueError("network parameter / normalisation size mismatch")
 
   @classmethod
-  def initial(cls, seed=0, device=None):
+  def initial(cls, seed=0, device=None, use_gap=True):
     """Glorot-uniform hidden layers, a small output layer (outputs start
     near 0: factor 1, amplitude 1), identity normalisation."""
     rng = np.random.default_rng(seed)
@@ -77,17 +79,18 @@ class PrecondNet:
     parts.append(np.zeros(2))
     p = np.concatenate([x.ravel() for x in parts]).astype(np.float32)
     norm = np.concatenate([np.zeros(nin), np.ones(nin)]).astype(np.float32)
-    return cls(p, norm, device)
+    return cls(p, norm, device, use_gap)
 
   @classmethod
   def load(cls, path=DEFAULT_MODEL, device=None):
     with np.load(path, allow_pickle=False) as z:
-      return cls(z["params"], z["norm"], device)
+      use_gap = bool(z["use_gap"]) if "use_gap" in z else False
+      return cls(z["params"], z["norm"], device, use_gap)
 
   def save(self, path, **meta):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez(path, params=self.params.cpu().numpy(),
-             norm=self.norm.cpu().numpy(),
+             norm=self.norm.cpu().numpy(), use_gap=np.int32(self.use_gap),
              **{k: np.asarray(v) for k, v in meta.items()})
 
   def forward(self, x):
@@ -111,24 +114,39 @@ def density_stats(plan: EnergyPlan, out=None):
   return out
 
 
-def coefficient_features(plan: EnergyPlan, idx, stats):
+def band_gaps(plan: EnergyPlan):
+  """Per band (s,k,b), from the last evaluation with a gradient: the
+  Rayleigh quotient of the highest occupied band of its (s,k) block minus
+  its own (Q^H Qbar has diagonal 2 f eps_b); 0 for empty bands.  Device
+  float64 tensor."""
+  d = torch.real(torch.diagonal(plan.subspace_matrix(), dim1=-2, dim2=-1))
+  occ = torch.as_tensor(plan.occupation, device=plan.device)
+  occupied = occ > 0
+  eps = torch.where(occupied, d / (2 * torch.where(occupied, occ, 1.0)),
+                    torch.full_like(d, -np.inf))
+  top = eps.max(dim=-1, keepdim=True).values
+  return torch.where(occupied, top - eps, torch.zeros_like(d)).contiguous()
+
+
+def coefficient_features(plan: EnergyPlan, idx, stats, gap=None):
   """Raw features (n, 8) float32 of flat coefficient indices idx."""
   idx = torch.as_tensor(idx, dtype=torch.int64).to(plan.device).contiguous()
   x = torch.empty((idx.numel(), NFEAT), dtype=torch.float32,
                   device=plan.device)
   _lib.check(_lib.lib().jrx_plan_features(
-    plan._plan, _lib.ptr(idx), idx.numel(), _lib.ptr(stats), _lib.ptr(x),
-    plan._ws_ptr, plan.workspace_bytes, _lib.stream_ptr(plan.device)))
+    plan._plan, _lib.ptr(idx), idx.numel(), _lib.ptr(stats), _lib.ptr(gap),
+    _lib.ptr(x), plan._ws_ptr, plan.workspace_bytes,
+    _lib.stream_ptr(plan.device)))
   return x
 
 
-def apply(plan: EnergyPlan, net: PrecondNet, stats, factor=None, w_re=None,
-          w_im=None):
+def apply(plan: EnergyPlan, net: PrecondNet, stats, gap=None, factor=None,
+          w_re=None, w_im=None):
   """Per-coefficient inference: factor = exp(y0), w *= exp(y1)."""
   _lib.check(_lib.lib().jrx_plan_mlp_apply(
     plan._plan, _lib.ptr(net.params), _lib.ptr(net.norm), _lib.ptr(stats),
-    _lib.ptr(factor), _lib.ptr(w_re), _lib.ptr(w_im), plan._ws_ptr,
-    plan.workspace_bytes, _lib.stream_ptr(plan.device)))
+    _lib.ptr(gap), _lib.ptr(factor), _lib.ptr(w_re), _lib.ptr(w_im),
+    plan._ws_ptr, plan.workspace_bytes, _lib.stream_ptr(plan.device)))
 
 
 def warm_start(plan: EnergyPlan, net: PrecondNet, params):
@@ -159,7 +177,8 @@ class LearnedFactor:
   def __call__(self, minimizer):
     if minimizer.restarts != self._stamp:
       density_stats(self.plan, self.stats)
-      apply(self.plan, self.net, self.stats, factor=self.factor)
+      gap = band_gaps(self.plan) if self.net.use_gap else None
+      apply(self.plan, self.net, self.stats, gap=gap, factor=self.factor)
       self._stamp = minimizer.restarts
     return self.factor
 
@@ -190,6 +209,21 @@ class Dataset:
   def save(self, path):
     np.savez(path, x_pc=self.x_pc, a=self.a, b=self.b, t2=self.t2,
              snap=self.snap, x_ws=self.x_ws, y_ws=self.y_ws)
+
+  def split(self, frac: float, seed: int = 0):
+    """(train, validation): a random fraction of the snapshots and of the
+    warm-start samples held out."""
+    rng = np.random.default_rng(seed)
+    ns = len(self.t2)
+    val = np.zeros(ns, bool)
+    val[rng.permutation(ns)[:max(1, int(round(frac * ns)))]] = True
+    rows = np.repeat(val, self.snap)
+    wval = rng.random(len(self.y_ws)) < frac
+
+    def pick(m, r, w):
+      return Dataset(self.x_pc[r], self.a[r], self.b[r], self.t2[m],
+                     self.snap, self.x_ws[w], self.y_ws[w])
+    return pick(~val, ~rows, ~wval), pick(val, rows, wval)
 
   @staticmethod
   def load(path):
@@ -223,7 +257,7 @@ def grassmann_direction(q_t, q_star):
 
 def collect_trajectory(wl, seed: int, snap_every: int = 8,
                        samples: int = 512, ws_samples: int = 4096,
-                       max_iter: int = 400, max_snaps: int = 24,
+                       max_iter: int = 800, max_snaps: int = 60,
                        rng: Optional[np.random.Generator] = None,
                        device=None) -> Dataset:
   """One plain-method trajectory of workload wl from wl.params(seed), turned
@@ -258,7 +292,8 @@ def collect_trajectory(wl, seed: int, snap_every: int = 8,
     h = (m.h[0].reshape(-1)[torch.as_tensor(idx, device=plan.device)].cpu()
          .numpy() + 1j * m.h[1].reshape(-1)[torch.as_tensor(
            idx, device=plan.device)].cpu().numpy())
-    x = coefficient_features(plan, idx, density_stats(plan)).cpu().numpy()
+    x = coefficient_features(plan, idx, density_stats(plan),
+                             band_gaps(plan)).cpu().numpy()
     snaps.append((q, w, idx, h, x))
 
   m = Minimizer(plan, p0)
@@ -373,19 +408,49 @@ class Trainer:
     return lpc, lws
 
 
-def train(ds: Dataset, epochs=400, lr=2e-3, seed=0, ws_weight=0.1,
-          device=None, log_every=0):
-  """Train a fresh network on ds; returns (net, history of (cos, mse))."""
-  net = PrecondNet.initial(seed, device)
+def train(ds: Dataset, epochs=1500, lr=2e-3, lr_end=5e-5, seed=0,
+          ws_weight=0.1, val_frac=0.1, use_gap=True, device=None,
+          log_every=0):
+  """Train a fresh network on ds with Adam and a cosine learning-rate
+  schedule lr -> lr_end; the parameters of the epoch with the lowest
+  validation loss (cosine + ws_weight * mse on a held-out fraction of
+  snapshots and warm-start samples) are returned.  Returns (net, history
+  of (epoch, train cos, train mse, val cos, val mse))."""
+  if not use_gap:  # the band-gap slot reads 0, as at inference
+    ds = Dataset(ds.x_pc.copy(), ds.a, ds.b, ds.t2, ds.snap, ds.x_ws,
+                 ds.y_ws)
+    ds.x_pc[:, 5] = 0.0
+  net = PrecondNet.initial(seed, device, use_gap)
   net.norm.copy_(torch.as_tensor(feature_norm(ds)))
-  tr = Trainer(net, ds, lr=lr, ws_weight=ws_weight)
+  if val_frac > 0 and len(ds.t2) >= 10:
+    ds_tr, ds_val = ds.split(val_frac, seed)
+  else:
+    ds_tr, ds_val = ds, ds
+  tr = Trainer(net, ds_tr, lr=lr, ws_weight=ws_weight)
+  ev = Trainer(net, ds_val, lr=lr, ws_weight=ws_weight)
   hist = []
+  best, best_params = np.inf, net.params.clone()
+
+  def validate(ep, lpc, lws):
+    nonlocal best
+    vpc, vws = ev.loss_and_grad()
+    vpc, vws = float(vpc), float(vws)
+    score = vpc + ws_weight * vws
+    if score < best:
+      best = score
+      best_params.copy_(net.params)
+    hist.append((ep, float(lpc), float(lws), vpc, vws))
+    return vpc, vws
+
   for ep in range(epochs):
+    tr.lr = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * ep / epochs))
     lpc, lws = tr.step()
-    if log_every and (ep % log_every == 0 or ep == epochs - 1):
-      hist.append((ep, float(lpc), float(lws)))
-      print(f"epoch {ep}: cos-loss {hist[-1][1]:.5f} ws-mse {hist[-1][2]:.4f}",
-            flush=True)
+    if ep % 10 == 0 or ep == epochs - 1:
+      vpc, vws = validate(ep, lpc, lws)
+      if log_every and (ep % log_every == 0 or ep == epochs - 1):
+        print(f"epoch {ep}: train cos {float(lpc):.5f} mse {float(lws):.4f}"
+              f" | val cos {vpc:.5f} mse {vws:.4f}", flush=True)
   lpc, lws = tr.loss_and_grad()
-  hist.append((epochs, float(lpc), float(lws)))
+  validate(epochs, lpc, lws)
+  net.params.copy_(best_params)
   return net, hist

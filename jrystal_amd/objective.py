@@ -39,11 +39,13 @@ class EnergyPlan:
     positions: (natom, 3) bohr.  charges: (natom,).  cell_vectors: (3, 3)
     bohr, rows.  mask: (nx, ny, nz) bool.  kpts: (nk, 3).  occupation:
     (ns, nk, nb).  chunk_states: states per FFT chunk (0 = automatic).
+    fft_variant: 0 automatic, 1 runtime-radix FFT kernels everywhere.
+    xline_blocks: target workgroups of an x-line pass (0 = 4096).
   """
 
   def __init__(self, positions, charges, cell_vectors, mask, kpts,
                occupation, xc: str = "lda_x", chunk_states: int = 0,
-               device=None):
+               device=None, fft_variant: int = 0, xline_blocks: int = 0):
     _require_cuda()
     if xc != "lda_x":
       raise NotImplementedError(f"xc {xc!r}: only lda_x is on this path")
@@ -77,6 +79,11 @@ class EnergyPlan:
     d.occupation = occ.ctypes.data
     d.xc = 0
     d.chunk_states = int(chunk_states)
+    d.fft_variant = int(fft_variant)
+    d.xline_blocks = int(xline_blocks)
+    self.options = {"chunk_states": int(chunk_states),
+                    "fft_variant": int(fft_variant),
+                    "xline_blocks": int(xline_blocks)}
     h = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(self._lib.jrx_plan_create(ctypes.byref(h), ctypes.byref(d)))
@@ -98,10 +105,10 @@ class EnergyPlan:
     self._ws_ptr = (self._ws.data_ptr() + 255) // 256 * 256
 
   @classmethod
-  def from_workload(cls, wl, chunk_states=0, device=None):
+  def from_workload(cls, wl, chunk_states=0, device=None, **options):
     return cls(wl.crystal.positions, wl.crystal.charges,
                wl.crystal.cell_vectors, wl.mask, wl.kpts, wl.occupation,
-               chunk_states=chunk_states, device=device)
+               chunk_states=chunk_states, device=device, **options)
 
   def close(self):
     if getattr(self, "_plan", None) is not None and self._plan.value:

@@ -9,8 +9,10 @@ Random cells (numpy.random.default_rng(seed)):
     symmetric strain of amplitude 0.1, atoms displaced by N(0, 0.15) bohr,
     Z in {3, 4}; 32^3 grid, 2x2x2 k-mesh, 8 bands, cutoff 24 Ha;
   * cube8 -- 8-atom diamond cubes, a = 10.26 bohr, strain amplitude 0.1,
-    displacements N(0, 0.15) bohr, Z = 4; 64^3 grid, Gamma, 32 bands,
-    cutoff 48 Ha.
+    displacements N(0, 0.15) bohr, Z = 4; 64^3 grid, Gamma or a 2x2x2
+    k-mesh, 32 bands, cutoff 48 Ha.
+Mixes of kinds: "small" cycles fcc2, fcc2, fcc2, cube8 (Gamma); "wide"
+cycles fcc2, fcc2, cube8 (Gamma), cube8 (2x2x2).
 Cells whose minimum-image atom separation is below 2.5 bohr are redrawn.
 Trajectory seeds start at 1000, away from the strained held-out cell
 (strain seed 4).
@@ -39,7 +41,7 @@ def _min_separation(cell, pos):
   return best
 
 
-def random_cell(seed: int, kind: str):
+def random_cell(seed: int, kind: str, k: int = 1):
   rng = np.random.default_rng(seed)
   while True:
     strain = W.strain_matrix(int(rng.integers(1 << 30)), 0.1)
@@ -55,7 +57,7 @@ def random_cell(seed: int, kind: str):
       base = W.diamond_cube(z)
       cell = base.cell_vectors @ (np.eye(3) + strain)
       frac = base.positions @ np.linalg.inv(base.cell_vectors)
-      grid, kgrid, nb, ecut = (64,) * 3, (1, 1, 1), 32, 48.0
+      grid, kgrid, nb, ecut = (64,) * 3, (k, k, k), 32, 48.0
     pos = frac @ cell + rng.normal(0, 0.15, (len(frac), 3))
     if _min_separation(cell, pos) < 2.5:
       continue
@@ -64,15 +66,24 @@ def random_cell(seed: int, kind: str):
     return W.build(f"{kind}-{seed}", cr, grid, kgrid, nb, ecut)
 
 
-def generate(n: int, seconds: float, seed0: int = 1000, cube_every: int = 4,
-             log=print):
+MIXES = {
+  "small": (("fcc2", 1), ("fcc2", 1), ("fcc2", 1), ("cube8", 1)),
+  "wide": (("fcc2", 1), ("fcc2", 1), ("cube8", 1), ("cube8", 2)),
+}
+
+
+def generate(n: int, seconds: float, seed0: int = 1000, mix: str = "small",
+             max_snaps: int = 24, max_iter: int = 400, log=print):
   parts, t0 = [], time.perf_counter()
+  kinds = MIXES[mix]
   for i in range(n):
     if time.perf_counter() - t0 > seconds:
       break
-    kind = "cube8" if i % cube_every == cube_every - 1 else "fcc2"
-    wl = random_cell(seed0 + i, kind)
-    parts.append(learned.collect_trajectory(wl, seed=seed0 + i))
+    kind, k = kinds[i % len(kinds)]
+    wl = random_cell(seed0 + i, kind, k)
+    parts.append(learned.collect_trajectory(wl, seed=seed0 + i,
+                                            max_snaps=max_snaps,
+                                            max_iter=max_iter))
     if log and (i % 20 == 0):
       log(f"trajectory {i} ({kind}): {len(parts[-1].t2)} snapshots, "
           f"{time.perf_counter() - t0:.1f} s")
@@ -85,20 +96,30 @@ def main(argv=None):
   ap.add_argument("--data", default=None, help="save / reuse the dataset")
   ap.add_argument("--trajectories", type=int, default=2000)
   ap.add_argument("--seconds", type=float, default=600.0)
-  ap.add_argument("--epochs", type=int, default=600)
+  ap.add_argument("--epochs", type=int, default=1500)
   ap.add_argument("--lr", type=float, default=2e-3)
+  ap.add_argument("--lr-end", type=float, default=5e-5)
+  ap.add_argument("--val-frac", type=float, default=0.1)
+  ap.add_argument("--mix", default="small", choices=sorted(MIXES))
+  ap.add_argument("--max-snaps", type=int, default=24)
+  ap.add_argument("--max-iter", type=int, default=400)
+  ap.add_argument("--no-gap", action="store_true",
+                  help="train without the band-gap feature")
   args = ap.parse_args(argv)
   import os
   if args.data and os.path.exists(args.data):
     ds = learned.Dataset.load(args.data)
     ntraj = -1
   else:
-    ds, ntraj = generate(args.trajectories, args.seconds)
+    ds, ntraj = generate(args.trajectories, args.seconds, mix=args.mix,
+                         max_snaps=args.max_snaps, max_iter=args.max_iter)
     if args.data:
       ds.save(args.data)
   print(f"dataset: {ntraj} trajectories, {len(ds.t2)} snapshots, "
         f"{ds.x_ws.shape[0]} warm-start samples", flush=True)
   net, hist = learned.train(ds, epochs=args.epochs, lr=args.lr,
+                            lr_end=args.lr_end, val_frac=args.val_frac,
+                            use_gap=not args.no_gap,
                             log_every=max(1, args.epochs // 10))
   net.save(args.out, trajectories=ntraj, snapshots=len(ds.t2),
            history=np.asarray(hist))

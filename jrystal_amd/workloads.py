@@ -108,6 +108,61 @@ def si8_strained(z=4.0, k=4, seed=4, amplitude=0.1):
   return wl
 
 
+SMOOTH_32_80 = tuple(n for n in range(32, 81) if _grid.fft_factor(n) == n)
+
+
+def alias_free_cutoff(cell, n):
+  """Largest integer cutoff (Ha) whose G-sphere fits |m_d| <= n_d / 4 (the
+  alias-free half box of grid.cubic_mask): planes of constant m_d lie
+  2 pi / |a_d| apart."""
+  cell = np.asarray(cell, float)
+  g = min((nd // 4) * 2 * np.pi / np.linalg.norm(cell[d])
+          for d, nd in enumerate(n))
+  return float(np.floor(0.5 * g * g))
+
+
+def batch_cells(seed=3, n=16):
+  """Uneven structures for the batched step: numpy.random.default_rng(seed)
+  draws, per structure, a cubic or FCC cell of 2-16 atoms (volume per atom
+  U(60, 140) bohr^3), Z from {3, 6, 11, 13, 14}, fractional positions
+  U[0,1) redrawn until every minimum-image separation is >= 2.5 bohr, a
+  cubic FFT grid from the 7-smooth sizes in [32, 80], a k-mesh of 1^3 to
+  4^3, 4-48 bands and the alias-free cutoff of the grid.  Occupations are
+  uniform (clipped at 2 per band, as in the reference)."""
+  rng = np.random.default_rng(seed)
+  out = []
+  for i in range(n):
+    lattice = ("cubic", "fcc")[int(rng.integers(2))]
+    natom = int(rng.integers(2, 17))
+    z = float(rng.choice([3.0, 6.0, 11.0, 13.0, 14.0]))
+    vol = natom * rng.uniform(60.0, 140.0)
+    if lattice == "cubic":
+      cell = vol**(1 / 3) * np.eye(3)
+    else:
+      a = (4 * vol)**(1 / 3)
+      cell = a / 2 * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0.]])
+    shifts = np.array([[i1, i2, i3] for i1 in (-1, 0, 1) for i2 in (-1, 0, 1)
+                       for i3 in (-1, 0, 1)], float) @ cell
+    while True:
+      frac = rng.random((natom, 3))
+      pos = frac @ cell
+      d = pos[None, :, None, :] - pos[:, None, None, :] + shifts[None, None]
+      r = np.linalg.norm(d, axis=-1)
+      r[np.arange(natom), np.arange(natom), 13] = np.inf  # self, no shift
+      if r.min() >= 2.5:
+        break
+    ng = int(rng.choice(SMOOTH_32_80))
+    k = int(rng.integers(1, 5))
+    nb = int(rng.integers(4, 49))
+    gs = (ng, ng, ng)
+    cr = Crystal(charges=np.full(natom, z), positions=pos, cell_vectors=cell,
+                 spin=0, symbols=["X"] * natom)
+    out.append(build(f"batch{seed}-{i}-{lattice}{natom}-z{int(z)}-{ng}^3-"
+                     f"{k**3}k-{nb}b", cr, gs, (k, k, k), nb,
+                     alias_free_cutoff(cell, gs)))
+  return out
+
+
 def si64(z=4.0):
   return build("si64-128^3-gamma-128b", diamond_cube(z, reps=2), (128,) * 3,
                (1, 1, 1), 128, 48.0)

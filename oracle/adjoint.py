@@ -102,8 +102,9 @@ def qr_backward(q, r, qbar):
 
 
 def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
-                    want_grad: bool = True):
-  """Returns (energies dict incl. 'total', grad dict or None, rho)."""
+                    want_grad: bool = True, extras: bool = False):
+  """Returns (energies dict incl. 'total', grad dict or None, rho); with
+  extras, a fourth item {'q': Q, 'qbar': Qbar} (both (s,k,g,b))."""
   w = params["w_re"] + 1j * params["w_im"]
   q, r = np.linalg.qr(w, mode="reduced")  # (s,k,g,b), (s,k,b,b)
   ns, nk, ng, nb = q.shape
@@ -144,7 +145,10 @@ def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
       qbar[s, k0:k1] = f * (2.0 / np.sqrt(vol) * hv +
                             gk2[k0:k1, :, None] * q[s, k0:k1])
   wbar = qr_backward(q, r, qbar)
-  return parts, {"w_re": np.real(wbar), "w_im": np.imag(wbar)}, rho
+  grad = {"w_re": np.real(wbar), "w_im": np.imag(wbar)}
+  if extras:
+    return parts, grad, rho, {"q": q, "qbar": qbar}
+  return parts, grad, rho
 
 
 def total_energy_literal(params, prob: Problem):

@@ -29,10 +29,12 @@ def unpack(p):
           p[B2:W3], p[W3:B3].reshape(OUT, HID), p[B3:NPARAMS])
 
 
-def features(gk2, tb, rho_mean, rho_var):
+def features(gk2, tb, rho_mean, rho_var, gap=0.0):
   """Raw features (..., 8): log1p(|G+k|^2), log1p(|G+k|^2/tb), log tb,
-  log rho_mean, log1p(rho_var / rho_mean^2), 0, 0, 0."""
-  gk2, tb = np.broadcast_arrays(np.asarray(gk2, float), np.asarray(tb, float))
+  log rho_mean, log1p(rho_var / rho_mean^2), log1p(gap), 0, 0."""
+  gk2, tb, gap = np.broadcast_arrays(np.asarray(gk2, float),
+                                     np.asarray(tb, float),
+                                     np.asarray(gap, float))
   t = np.maximum(tb, 1e-300)
   m = max(float(rho_mean), 1e-300)
   out = np.zeros(gk2.shape + (IN,))
@@ -41,7 +43,19 @@ def features(gk2, tb, rho_mean, rho_var):
   out[..., 2] = np.log(t)
   out[..., 3] = np.log(m)
   out[..., 4] = np.log1p(max(float(rho_var), 0.0) / (m * m))
+  out[..., 5] = np.log1p(np.maximum(gap, 0.0))
   return out
+
+
+def band_gaps(subspace, occupation):
+  """Per band (s,k,b): Rayleigh quotient of the highest occupied band of
+  its (s,k) block minus its own, from Q^H Qbar (diagonal 2 f eps_b); 0 for
+  empty bands."""
+  occ = np.asarray(occupation, float)
+  d = np.real(np.diagonal(subspace, axis1=-2, axis2=-1))
+  eps = np.where(occ > 0, d / (2 * np.where(occ > 0, occ, 1.0)), -np.inf)
+  top = np.max(eps, axis=-1, keepdims=True)
+  return np.where(occ > 0, top - eps, 0.0)
 
 
 def forward(p, norm, x, keep=False):

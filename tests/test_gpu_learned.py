@@ -124,12 +124,25 @@ def test_density_stats_features_and_inference_match_oracle():
   gm = prob.mask_g()
   gk2 = np.sum((gm[None] + prob.kpts[:, None, :])**2, axis=-1)
   ns, nk, ng, nb = plan.param_shape
-  feats_ref = OM.features(gk2[None, :, :, None], tb[:, :, None, :], *stats)
+  # band gaps from Q^H Qbar, against the oracle's Q and Qbar
+  _, _, _, ex = OA.energy_and_grad(p, prob, extras=True)
+  d = np.diagonal(np.linalg.qr(p["w_re"] + 1j * p["w_im"])[1], axis1=-2,
+                  axis2=-1)
+  ph = d / np.abs(d)  # column phases of the positive-diagonal QR
+  sub = np.conj(np.swapaxes(ex["q"] * ph[..., None, :], -1, -2)) @ (
+    ex["qbar"] * ph[..., None, :])
+  gap_ref = OM.band_gaps(sub, plan.occupation)
+  gap = learned.band_gaps(plan)
+  assert rel_max(gap.cpu().numpy(), gap_ref) < 1e-9
+  assert np.all(gap_ref[..., 4:] == 0) and np.all(gap_ref[..., :4] >= 0)
+  feats_ref = OM.features(gk2[None, :, :, None], tb[:, :, None, :], *stats,
+                          gap=gap_ref[:, :, None, :])
   feats_ref = feats_ref.reshape(-1, 8)
   rng = np.random.default_rng(0)
   idx = rng.integers(0, ns * nk * ng * nb, 3000)
   idx[:2] = [-1, ns * nk * ng * nb]  # out of range -> zeros
-  x = learned.coefficient_features(plan, idx, learned.density_stats(plan))
+  x = learned.coefficient_features(plan, idx, learned.density_stats(plan),
+                                   gap)
   x = x.cpu().numpy()
   assert np.all(x[:2] == 0)
   np.testing.assert_allclose(x[2:], feats_ref[idx[2:]], rtol=1e-6, atol=1e-6)
@@ -139,7 +152,7 @@ def test_density_stats_features_and_inference_match_oracle():
   fac = torch.empty(plan.param_shape, dtype=torch.float64, device="cuda")
   wr = torch.tensor(p["w_re"], device="cuda")
   wi = torch.tensor(p["w_im"], device="cuda")
-  learned.apply(plan, net, learned.density_stats(plan), factor=fac,
+  learned.apply(plan, net, learned.density_stats(plan), gap=gap, factor=fac,
                 w_re=wr, w_im=wi)
   y = OM.forward(par, norm, feats_ref)
   f_ref = np.exp(np.clip(y[:, 0], -8, 8)).reshape(plan.param_shape)
@@ -183,6 +196,9 @@ def test_trajectory_collection_and_training_reduce_loss():
                                   ws_samples=1024, max_iter=120)
   assert len(ds.t2) >= 3 and ds.x_pc.shape == (len(ds.t2) * 256, 8)
   assert np.all(np.isfinite(ds.a)) and np.all(ds.b >= 0)
-  net, hist = learned.train(ds, epochs=60, lr=3e-3)
-  net0, hist0 = learned.train(ds, epochs=0)
+  net, hist = learned.train(ds, epochs=60, lr=3e-3, val_frac=0)
+  net0, hist0 = learned.train(ds, epochs=0, val_frac=0)
   assert hist[-1][1] < hist0[-1][1] and hist[-1][2] < hist0[-1][2]
+  tr, va = ds.split(0.25, seed=1)
+  assert len(tr.t2) + len(va.t2) == len(ds.t2) and len(va.t2) >= 1
+  assert tr.x_pc.shape[0] == len(tr.t2) * ds.snap

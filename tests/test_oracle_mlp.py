@@ -108,3 +108,5 @@ def test_features_closed_form():
   np.testing.assert_allclose(f[1, :5], [np.log(4.0), np.log(2.5), np.log(2.0),
                                         np.log(0.5), np.log(2.0)], rtol=1e-15)
   assert np.all(f[:, 5:] == 0) and f[0, 0] == 0 and f[0, 1] == 0
+  g = OM.features(np.array([1.0]), np.array([1.0]), 1.0, 0.0, gap=np.e - 1)
+  assert abs(g[0, 5] - 1.0) < 1e-15
