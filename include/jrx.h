@@ -71,6 +71,8 @@ typedef struct {
   int chunk_states;        /* states per FFT chunk, 0 = automatic           */
   int fft_variant;         /* 0 = automatic, 1 = runtime-radix kernels only */
   int xline_blocks;        /* target workgroups of an x-line pass, 0 = 4096 */
+  int plane_pad;           /* extra elements per x-plane of the plane
+                              buffer, -1 = automatic                        */
 } jrx_problem;
 
 int jrx_plan_create(jrx_plan** plan, const jrx_problem* problem);

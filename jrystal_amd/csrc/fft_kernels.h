@@ -17,6 +17,9 @@ struct BoxGeom {
   const int* row_off;   // [nxa+1] offsets into rows
   int max_rows;         // max active rows over planes
   const int* xmap;      // [nx]    active-plane index of x, or -1
+  int ps;               // element stride of the x-planes of the plane
+                        // buffer T (ny*nz plus a pad that spreads the planes
+                        // read together by the x-line passes over channels)
 };
 
 // y = scale * DFT_axis(x) along one axis of a (B, n0, n1, n2) complex box.

@@ -263,7 +263,18 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     return JRX_ERR_UNSUPPORTED;
   }
   // Chunking of the FFT sweeps.
-  const size_t per_state = (size_t)p->geo.nxa * ny * nz * sizeof(double2);
+  // Plane stride of the plane buffer: ny*nz rounded up to 16 elements
+  // (256 B), plus 16 when that is a multiple of 256 elements, so that the
+  // x-planes an x-line pass reads together are not 2^k bytes apart.
+  {
+    const int64_t P2 = (int64_t)ny * nz;
+    int64_t ps = (P2 + 15) / 16 * 16;
+    if (d->plane_pad >= 0) ps = P2 + d->plane_pad;
+    else if (ps % 256 == 0) ps += 16;
+    JRX_REQUIRE(ps < (int64_t)1 << 30, "plane too large");
+    p->geo.ps = (int)ps;
+  }
+  const size_t per_state = (size_t)p->geo.nxa * p->geo.ps * sizeof(double2);
   // Chunking of the FFT sweeps.  When the plane buffer can hold every state
   // (default budget 16 GiB), the forward sweep's planes are kept and the
   // adjoint sweep starts from them instead of transforming Q again.
@@ -852,7 +863,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   // --- density -------------------------------------------------------------
   const int64_t SS = (int64_t)p->nk * nb;  // states per spin
   const bool reg64 = use_reg64(p->geo) && !p->force_generic;
-  const int64_t plane_elems = (int64_t)p->geo.nxa * p->ny * p->nz;
+  const int64_t plane_elems = (int64_t)p->geo.nxa * p->geo.ps;
   for (int sp = 0; sp < p->ns; ++sp) {
     JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
     const int64_t step = p->keep_planes ? SS : p->chunk;

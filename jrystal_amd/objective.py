@@ -41,11 +41,14 @@ class EnergyPlan:
     (ns, nk, nb).  chunk_states: states per FFT chunk (0 = automatic).
     fft_variant: 0 automatic, 1 runtime-radix FFT kernels everywhere.
     xline_blocks: target workgroups of an x-line pass (0 = 4096).
+    plane_pad: extra elements per x-plane of the plane buffer (-1 =
+    automatic).
   """
 
   def __init__(self, positions, charges, cell_vectors, mask, kpts,
                occupation, xc: str = "lda_x", chunk_states: int = 0,
-               device=None, fft_variant: int = 0, xline_blocks: int = 0):
+               device=None, fft_variant: int = 0, xline_blocks: int = 0,
+               plane_pad: int = -1):
     _require_cuda()
     if xc != "lda_x":
       raise NotImplementedError(f"xc {xc!r}: only lda_x is on this path")
@@ -81,9 +84,11 @@ class EnergyPlan:
     d.chunk_states = int(chunk_states)
     d.fft_variant = int(fft_variant)
     d.xline_blocks = int(xline_blocks)
+    d.plane_pad = int(plane_pad)
     self.options = {"chunk_states": int(chunk_states),
                     "fft_variant": int(fft_variant),
-                    "xline_blocks": int(xline_blocks)}
+                    "xline_blocks": int(xline_blocks),
+                    "plane_pad": int(plane_pad)}
     h = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(self._lib.jrx_plan_create(ctypes.byref(h), ctypes.byref(d)))
