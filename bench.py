@@ -147,7 +147,8 @@ def convergence(time_cap, tol=1e-6):
     if mode == "learned_warm":
       p0 = learned.warm_start(plan, net, p0)
     fn = None if mode == "plain" else learned.LearnedFactor(plan, net)
-    m = Minimizer(plan, p0, factor_fn=fn)
+    m = Minimizer(plan, p0, factor_fn=fn, restart=(
+      60 if mode == "plain" else learned.LEARNED_RESTART))
     t_setup = time.perf_counter() - t0
     stop = None if e_best is None else e_best + 0.1 * tol
     r = m.run(max_iter=100000, time_limit=time_cap, e_stop=stop)

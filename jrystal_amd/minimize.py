@@ -210,6 +210,20 @@ class Minimizer:
     for it in range(1, max_iter + 1):
       self._dot(self.g, self.d, 0)
       s0 = float(self.dotv[0])
+      if not (np.isfinite(s0) and np.isfinite(gh)):
+        # a preconditioner factor that is not finite: plain preconditioner
+        fallbacks += 1
+        stalled += 1
+        self._plain_cycles = 2
+        self._s0_prev = None
+        self._eval(self.W, self.e, self.g)
+        E0, gh = self._restart()
+        since_restart = 0
+        hist.append((time.perf_counter() - t0, E0))
+        if stalled >= 3:
+          status = "stalled"
+          break
+        continue
       if not s0 < 0.0:  # not a descent direction: steepest descent
         self._axpby(-1.0, self.h, 0.0, self.h, self.d)
         s0 = -gh

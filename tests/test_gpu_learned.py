@@ -202,3 +202,54 @@ def test_trajectory_collection_and_training_reduce_loss():
   tr, va = ds.split(0.25, seed=1)
   assert len(tr.t2) + len(va.t2) == len(ds.t2) and len(va.t2) >= 1
   assert tr.x_pc.shape[0] == len(tr.t2) * ds.snap
+
+
+def _strained_cell():
+  from jrystal_amd import workloads as W
+  return W.si8_strained(4.0, k=2, seed=11)  # not the held-out strain (4)
+
+
+def test_trained_network_gains_over_plain_preconditioner():
+  """The shipped model against plain TPA on a strained 8-atom cell (8 k):
+  same minimum, fewer CG iterations to |dE| < 1e-6."""
+  from jrystal_amd import learned
+  from jrystal_amd.minimize import Minimizer
+  from jrystal_amd.objective import EnergyPlan
+  wl = _strained_cell()
+  plan = EnergyPlan.from_workload(wl)
+  net = learned.PrecondNet.load()
+  plain = Minimizer(plan, wl.params(0)).run(max_iter=2000)
+  lrn = Minimizer(plan, wl.params(0), factor_fn=learned.LearnedFactor(plan, net),
+                  restart=learned.LEARNED_RESTART).run(max_iter=2000)
+  e_star = min(plain.energy, lrn.energy)
+  assert abs(plain.energy - lrn.energy) < 1e-7
+
+  def its(r):
+    return next(i for i, (t, e) in enumerate(r.history) if e - e_star < 1e-6)
+  print(f"iterations to 1e-6: plain {its(plain)}, learned {its(lrn)}")
+  assert its(lrn) <= 0.8 * its(plain)
+  e = [h[1] for h in lrn.history]
+  assert all(b <= a + 1e-12 for a, b in zip(e, e[1:]))
+
+
+def test_fallback_to_plain_preconditioner():
+  """A factor that breaks the search direction (NaN) sends the minimiser
+  back to the plain preconditioner; it still reaches the plain minimum with
+  a monotone energy."""
+  from jrystal_amd.minimize import Minimizer
+  from jrystal_amd.objective import EnergyPlan
+  wl = _strained_cell()
+  plan = EnergyPlan.from_workload(wl)
+  bad = torch.full(plan.param_shape, float("nan"), dtype=torch.float64,
+                   device="cuda")
+
+  def factor_fn(m):
+    return bad if m.restarts >= 2 else None
+
+  ref = Minimizer(plan, wl.params(0)).run(max_iter=2000)
+  r = Minimizer(plan, wl.params(0), factor_fn=factor_fn).run(max_iter=4000)
+  assert r.fallbacks >= 1
+  assert abs(r.energy - ref.energy) < 1e-6
+  e = [h[1] for h in r.history]
+  assert all(np.isfinite(e))
+  assert all(b <= a + 1e-12 for a, b in zip(e, e[1:]))
