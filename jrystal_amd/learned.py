@@ -418,12 +418,14 @@ class Trainer:
 
 def train(ds: Dataset, epochs=1500, lr=2e-3, lr_end=5e-5, seed=0,
           ws_weight=0.1, val_frac=0.1, use_gap=True, device=None,
-          log_every=0):
+          log_every=0, checkpoint_every=0, checkpoints=None):
   """Train a fresh network on ds with Adam and a cosine learning-rate
   schedule lr -> lr_end; the parameters of the epoch with the lowest
   validation loss (cosine + ws_weight * mse on a held-out fraction of
   snapshots and warm-start samples) are returned.  Returns (net, history
-  of (epoch, train cos, train mse, val cos, val mse))."""
+  of (epoch, train cos, train mse, val cos, val mse)).  With
+  checkpoint_every > 0, (epoch, PrecondNet copy) pairs are appended to the
+  list `checkpoints` along the way."""
   if not use_gap:  # the band-gap slot reads 0, as at inference
     ds = Dataset(ds.x_pc.copy(), ds.a, ds.b, ds.t2, ds.snap, ds.x_ws,
                  ds.y_ws)
@@ -453,6 +455,11 @@ def train(ds: Dataset, epochs=1500, lr=2e-3, lr_end=5e-5, seed=0,
   for ep in range(epochs):
     tr.lr = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * ep / epochs))
     lpc, lws = tr.step()
+    if checkpoint_every and checkpoints is not None and \
+        (ep + 1) % checkpoint_every == 0:
+      checkpoints.append((ep + 1, PrecondNet(net.params.cpu().numpy(),
+                                             net.norm.cpu().numpy(),
+                                             net.device, net.use_gap)))
     if ep % 10 == 0 or ep == epochs - 1:
       vpc, vws = validate(ep, lpc, lws)
       if log_every and (ep % log_every == 0 or ep == epochs - 1):
@@ -462,3 +469,40 @@ def train(ds: Dataset, epochs=1500, lr=2e-3, lr_end=5e-5, seed=0,
   validate(epochs, lpc, lws)
   net.params.copy_(best_params)
   return net, hist
+
+
+def iterations_to(result, e_star, tol=1e-6):
+  """First CG iteration of a MinimizeResult within tol of e_star (None if
+  never)."""
+  return next((i for i, (t, e) in enumerate(result.history)
+               if e - e_star < tol), None)
+
+
+def select_by_convergence(nets, cells, tol=1e-6, max_iter=3000, log=None):
+  """Model selection on validation cells: for each network, the mean over
+  the cells of (CG iterations to |E - E*| < tol with the network's factor)
+  / (the same with the plain preconditioner).  Returns (best index,
+  scores)."""
+  from .minimize import Minimizer
+  per_cell = []
+  for wl in cells:
+    plan = EnergyPlan.from_workload(wl)
+    p0 = wl.params(0)
+    plain = Minimizer(plan, p0).run(max_iter=max_iter)
+    runs = []
+    for net in nets:
+      runs.append(Minimizer(plan, p0, factor_fn=LearnedFactor(plan, net),
+                            restart=LEARNED_RESTART).run(max_iter=max_iter))
+    e_star = min([plain.energy] + [r.energy for r in runs])
+    ip = iterations_to(plain, e_star, tol)
+    ratios = []
+    for r in runs:
+      i = iterations_to(r, e_star, tol)
+      ratios.append(np.inf if i is None or ip is None else i / ip)
+    per_cell.append(ratios)
+    if log:
+      log(f"{wl.name}: plain {ip} iterations; ratios "
+          + " ".join(f"{x:.3f}" for x in ratios))
+    plan.close()
+  scores = np.mean(np.asarray(per_cell), axis=0)
+  return int(np.argmin(scores)), scores

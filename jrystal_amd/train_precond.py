@@ -105,6 +105,15 @@ def main(argv=None):
   ap.add_argument("--max-iter", type=int, default=400)
   ap.add_argument("--no-gap", action="store_true",
                   help="train without the band-gap feature")
+  ap.add_argument("--both-features", action="store_true",
+                  help="train with and without the band-gap feature")
+  ap.add_argument("--select-cells", default="",
+                  help="strain seeds:k-mesh of validation cells, e.g. "
+                       "11:2,12:2,13:2; selects the checkpoint with the "
+                       "fewest CG iterations relative to plain TPA")
+  ap.add_argument("--checkpoint-every", type=int, default=100)
+  ap.add_argument("--also", default="",
+                  help="existing model files to include in the selection")
   args = ap.parse_args(argv)
   import os
   if args.data and os.path.exists(args.data):
@@ -117,12 +126,37 @@ def main(argv=None):
       ds.save(args.data)
   print(f"dataset: {ntraj} trajectories, {len(ds.t2)} snapshots, "
         f"{ds.x_ws.shape[0]} warm-start samples", flush=True)
-  net, hist = learned.train(ds, epochs=args.epochs, lr=args.lr,
-                            lr_end=args.lr_end, val_frac=args.val_frac,
-                            use_gap=not args.no_gap,
-                            log_every=max(1, args.epochs // 10))
-  net.save(args.out, trajectories=ntraj, snapshots=len(ds.t2),
-           history=np.asarray(hist))
+  variants = [not args.no_gap]
+  if args.both_features:
+    variants = [False, True]
+  cands, labels = [], []
+  net = hist = None
+  for use_gap in variants:
+    ckpts = []
+    net, hist = learned.train(ds, epochs=args.epochs, lr=args.lr,
+                              lr_end=args.lr_end, val_frac=args.val_frac,
+                              use_gap=use_gap,
+                              log_every=max(1, args.epochs // 10),
+                              checkpoint_every=(args.checkpoint_every
+                                                if args.select_cells else 0),
+                              checkpoints=ckpts)
+    cands += [net] + [c for _, c in ckpts]
+    labels += [f"gap={use_gap} best-val"] + \
+        [f"gap={use_gap} epoch {e}" for e, _ in ckpts]
+  for path in filter(None, args.also.split(",")):
+    cands.append(learned.PrecondNet.load(path))
+    labels.append(f"file {path}")
+  chosen, label = net, labels[0] if labels else ""
+  if args.select_cells:
+    cells = [W.si8_strained(4.0, k=int(k), seed=int(sd)) for sd, k in
+             (c.split(":") for c in args.select_cells.split(","))]
+    best, scores = learned.select_by_convergence(cands, cells, log=print)
+    for lab, sc in zip(labels, scores):
+      print(f"  {lab}: mean iteration ratio {sc:.3f}", flush=True)
+    chosen, label = cands[best], labels[best]
+    print(f"selected {label}", flush=True)
+  chosen.save(args.out, trajectories=ntraj, snapshots=len(ds.t2),
+              history=np.asarray(hist), selected=label)
   print(f"saved {args.out}", flush=True)
   return 0
 
