@@ -683,7 +683,7 @@ static size_t qr_ws_layout(int64_t P, int ng, int nb, size_t* o_part,
     off = (off + b + 255) & ~(size_t)255;
     return o;
   };
-  *o_part = take((size_t)P * sp.nchunk * 4 * nb * nb * sizeof(double2));
+  *o_part = take((size_t)P * sp.nchunk * nb * nb * sizeof(double2));
   *o_s = take(mat);
   *o_a = take(mat);
   *o_b = take(mat);
@@ -719,10 +719,10 @@ int jrx_qr(const double* w_re, const double* w_im, int64_t P, int ng, int nb,
   hipStream_t s = (hipStream_t)stream;
   int rc;
   if ((rc = launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, sp, s)) ||
-      (rc = launch_reduce_partials(part, S, P, 4 * sp.nchunk, nb, s)) ||
+      (rc = launch_reduce_partials(part, S, P, sp.nchunk, nb, s)) ||
       (rc = launch_chol_inv(S, R1, P, nb, s)) ||
       (rc = launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, sp, s)) ||
-      (rc = launch_reduce_partials(part, S, P, 4 * sp.nchunk, nb, s)) ||
+      (rc = launch_reduce_partials(part, S, P, sp.nchunk, nb, s)) ||
       (rc = launch_chol_inv(S, R2, P, nb, s)) ||
       (rc = launch_trmm_uu(R1, R2, (double2*)rinv, P, nb, s)) ||
       (rc = launch_apply_fwd(w_re, w_im, (const double2*)rinv, (double2*)q, P,
@@ -756,7 +756,7 @@ int jrx_qr_backward(const double* q, const double* qbar, const double* rinv,
   int rc;
   if ((rc = launch_gram_bmajor((const double2*)q, (const double2*)qbar, part,
                                P, ng, nb, sp, s)) ||
-      (rc = launch_reduce_partials(part, G, P, 4 * sp.nchunk, nb, s)) ||
+      (rc = launch_reduce_partials(part, G, P, sp.nchunk, nb, s)) ||
       (rc = launch_qr_bwd_small(G, (const double2*)rinv, A, Bm, P, nb, s)) ||
       (rc = launch_apply_bwd((const double2*)qbar, (const double2*)q, A, Bm,
                              dw_re, dw_im, P, ng, nb, s)))

@@ -252,7 +252,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   const size_t lds_plane = (size_t)ny * nz * sizeof(double2);
   const size_t nbp = 16 * (size_t)((nb + 15) / 16);
   const size_t lds_qr = 2 * nbp * nbp * sizeof(double2);
-  const size_t lds_gram = nbp * nbp * sizeof(double2);
+  const size_t lds_gram = 2 * nbp * nbp * sizeof(double2);  // pre + sums
   const size_t lds_chol = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
   size_t need = lds_plane;
   for (size_t v : {lds_qr, lds_gram, lds_chol}) need = v > need ? v : need;
@@ -328,7 +328,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_Ri = take(mat);
   p->o_A = take(mat);
   p->o_Bm = take(mat);
-  p->o_part = take((size_t)p->P * p->gsw.nchunk * 4 * nb * nb * sizeof(double2));
+  p->o_part = take((size_t)p->P * p->gsw.nchunk * nb * nb * sizeof(double2));
   p->o_kin = take((size_t)p->kin_parts_max * sizeof(double));
   p->o_tb = take((size_t)p->S * sizeof(double));
   p->o_eh = take((size_t)p->rec_parts * sizeof(double));
@@ -853,10 +853,10 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   } while (0)
   // --- QR (CholeskyQR2) ----------------------------------------------------
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, p->gsw, s));
-  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsw.nchunk, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s));
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s));
-  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsw.nchunk, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));
   JRX_T(JRX_K_QR_APPLY, launch_apply_fwd(w_re, w_im, Ri, q, P, ng, nb, s));
@@ -942,7 +942,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
     JRX_T(JRX_K_OTHER, launch_state_sums(kin, p->geo.nxa, S, tb, s));
     // --- QR adjoint ----------------------------------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
-    JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, 4 * p->gsq.nchunk, nb, s));
+    JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsq.nchunk, nb, s));
     JRX_T(JRX_K_QR_SMALL, launch_qr_bwd_small(Sm, Ri, Am, Bm, P, nb, s));
     JRX_T(JRX_K_QR_BWD_APPLY, launch_apply_bwd(qbar, q, Am, Bm, dw_re, dw_im, P, ng, nb, s));
   } else {

@@ -48,7 +48,7 @@ __device__ __forceinline__ void load_small(double2* dst, const double2* src,
   }
 }
 
-// Per-wave partial Gram sums over the chunk's rows:
+// Partial Gram sums over the chunk's rows (one per block):
 //   S = A^H A with A = W (PRE = false) or A = W pre (PRE = true).
 // Waves take 16-row slabs in turn; only tile pairs I <= J are accumulated
 // and the full Hermitian matrix is written.
@@ -126,27 +126,43 @@ __global__ __launch_bounds__(kQrBlock) void gram_w_kernel(
         }
     }
   }
-  double2* out = partial + ((p * nchunk + chunk) * 4 + wave) * nb * nb;
-  int pi = 0;
+  // Sum the four waves' tiles in LDS (wave order 0..3), one partial per
+  // block.
+  double2* red = sm + (PRE ? NBP * NBP : 0);
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+      int pi = 0;
 #pragma unroll
-  for (int I = 0; I < T; ++I)
+      for (int I = 0; I < T; ++I)
 #pragma unroll
-    for (int J = I; J < T; ++J, ++pi)
+        for (int J = I; J < T; ++J, ++pi)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * r + lk, j = 16 * J + li;
-        if (i < nb && j < nb) {
-          out[i * nb + j] = make_double2(sre[pi][r], sim[pi][r]);
-          if (I < J) out[j * nb + i] = make_double2(sre[pi][r], -sim[pi][r]);
-        }
-      }
+          for (int r = 0; r < 4; ++r) {
+            const int i = 16 * I + 4 * r + lk, j = 16 * J + li;
+            const double2 v = make_double2(sre[pi][r], sim[pi][r]);
+            if (w == 0) {
+              red[i * NBP + j] = v;
+              if (I < J) red[j * NBP + i] = cconj(v);
+            } else {
+              red[i * NBP + j] = cadd(red[i * NBP + j], v);
+              if (I < J) red[j * NBP + i] = cadd(red[j * NBP + i], cconj(v));
+            }
+          }
+    }
+    __syncthreads();
+  }
+  double2* out = partial + (p * nchunk + chunk) * nb * nb;
+  for (int e = threadIdx.x; e < nb * nb; e += kQrBlock) {
+    const int i = e / nb, j = e - i * nb;
+    out[e] = red[i * NBP + j];
+  }
 }
 
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
                        const GramSplit& sp, hipStream_t s) {
   const int T = (nb + 15) / 16;
-  size_t shm = pre ? (size_t)(16 * T) * (16 * T) * sizeof(double2) : 16;
+  size_t shm = (pre ? 2 : 1) * (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
 #define JRX_GRAM_W(TT)                                                        \
   if (pre)                                                                    \
@@ -170,7 +186,8 @@ int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
   return JRX_OK;
 }
 
-// Per-wave partial sums of G = Q^H Qbar over the chunk (band-major inputs).
+// Partial sums of G = Q^H Qbar over the chunk (band-major inputs), one per
+// block.
 template <int T>
 __global__ __launch_bounds__(kQrBlock) void gram_q_kernel(
     const double2* __restrict__ q, const double2* __restrict__ qb,
@@ -212,28 +229,43 @@ __global__ __launch_bounds__(kQrBlock) void gram_q_kernel(
         }
     }
   }
-  double2* out = partial + ((p * nchunk + chunk) * 4 + wave) * nb * nb;
+  // Sum the four waves' tiles in LDS (wave order 0..3), one partial per
+  // block.
+  extern __shared__ double2 red[];
+  constexpr int NBP = 16 * T;
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
 #pragma unroll
-  for (int I = 0; I < T; ++I)
+      for (int I = 0; I < T; ++I)
 #pragma unroll
-    for (int J = 0; J < T; ++J)
+        for (int J = 0; J < T; ++J)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * r + lk, j = 16 * J + li;
-        if (i < nb && j < nb) out[i * nb + j] = make_double2(gre[I][J][r], gim[I][J][r]);
-      }
+          for (int r = 0; r < 4; ++r) {
+            const int i = 16 * I + 4 * r + lk, j = 16 * J + li;
+            const double2 v = make_double2(gre[I][J][r], gim[I][J][r]);
+            red[i * NBP + j] = w == 0 ? v : cadd(red[i * NBP + j], v);
+          }
+    }
+    __syncthreads();
+  }
+  double2* out = partial + (p * nchunk + chunk) * nb * nb;
+  for (int e = threadIdx.x; e < nb * nb; e += kQrBlock) {
+    const int i = e / nb, j = e - i * nb;
+    out[e] = red[i * NBP + j];
+  }
 }
 
 int launch_gram_bmajor(const double2* q, const double2* qb, double2* partial,
                        int64_t P, int ng, int nb, const GramSplit& sp,
                        hipStream_t s) {
   const int T = (nb + 15) / 16;
+  const size_t shm = (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
   switch (T) {
-    case 1: hipLaunchKernelGGL((gram_q_kernel<1>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
-    case 2: hipLaunchKernelGGL((gram_q_kernel<2>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
-    case 3: hipLaunchKernelGGL((gram_q_kernel<3>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
-    case 4: hipLaunchKernelGGL((gram_q_kernel<4>), grid, block, 0, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 1: hipLaunchKernelGGL((gram_q_kernel<1>), grid, block, shm, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 2: hipLaunchKernelGGL((gram_q_kernel<2>), grid, block, shm, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 3: hipLaunchKernelGGL((gram_q_kernel<3>), grid, block, shm, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
+    case 4: hipLaunchKernelGGL((gram_q_kernel<4>), grid, block, shm, s, q, qb, partial, ng, nb, sp.rows_per_chunk, sp.nchunk); break;
     default:
       set_error("nb above 64 is not supported by this build");
       return JRX_ERR_UNSUPPORTED;

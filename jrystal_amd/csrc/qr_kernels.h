@@ -20,15 +20,15 @@ struct GramSplit {
 };
 GramSplit gram_split(int64_t P, int ng);
 
-// partial[p][4c + w] = sum over the rows of chunk c taken by wave w of
-// conj(a_g)^T a_g, a_g = W[p][g][:] (times pre[p] on the right when
-// pre != nullptr).  Four partials per chunk.
+// partial[p][c] = sum over the rows of chunk c of conj(a_g)^T a_g,
+// a_g = W[p][g][:] (times pre[p] on the right when pre != nullptr): one
+// partial per chunk (the block's four waves are summed in LDS).
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
                        const GramSplit& sp, hipStream_t s);
 
-// partial[p][4c + w] = sum over wave w's rows of chunk c of
-// conj(Q[p][:][g]) (x) Qb[p][:][g].
+// partial[p][c] = sum over the rows of chunk c of
+// conj(Q[p][:][g]) (x) Qb[p][:][g] (one partial per chunk).
 int launch_gram_bmajor(const double2* q, const double2* qb, double2* partial,
                        int64_t P, int ng, int nb, const GramSplit& sp,
                        hipStream_t s);

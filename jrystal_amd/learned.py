@@ -490,9 +490,11 @@ def select_by_convergence(nets, cells, tol=1e-6, max_iter=3000, log=None):
     p0 = wl.params(0)
     plain = Minimizer(plan, p0).run(max_iter=max_iter)
     runs = []
-    for net in nets:
+    for n, net in enumerate(nets):
       runs.append(Minimizer(plan, p0, factor_fn=LearnedFactor(plan, net),
                             restart=LEARNED_RESTART).run(max_iter=max_iter))
+      if log:
+        log(f"{wl.name}: candidate {n}: {runs[-1].iterations} iterations")
     e_star = min([plain.energy] + [r.energy for r in runs])
     ip = iterations_to(plain, e_star, tol)
     ratios = []
