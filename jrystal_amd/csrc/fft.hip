@@ -503,68 +503,90 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_ifft64_kernel(
   }
 }
 
+// Persistent: block b handles items b, b + gridDim.x, ... (item = state *
+// nxa + active plane) and loads the next item's columns from the plane
+// buffer while it transforms and gathers the current one (only two blocks
+// fit a CU's LDS, so in-block prefetch is what hides the load latency;
+// measured 7% faster than one block per item).
 __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather64_kernel(
     const double2* __restrict__ T, const double2* __restrict__ q,
     double2* __restrict__ qbar, double* __restrict__ kin_partial,
-    int64_t state0, BoxGeom geo, const double2* __restrict__ tw64,
-    GatherParams gp) {
+    int64_t state0, int nstates, BoxGeom geo,
+    const double2* __restrict__ tw64, GatherParams gp) {
   extern __shared__ double2 sm[];
   double2* pl = sm;
   double2* stw = sm + 64 * kPL;
   const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
-  const int xi = blockIdx.x, c = blockIdx.y;
-  const int64_t st = state0 + c;
+  const int nitems = geo.nxa * nstates;
   if (tid < 64) stw[tid] = tw64[tid];
   __syncthreads();
-  {  // y transform of column z = line, loaded straight from T
-    const double2* src = T + ((int64_t)c * geo.nxa + xi) * geo.ps + line;
+  int item = blockIdx.x;
+  double2 nv[8];
+  if (item < nitems) {
+    const int xi0 = item % geo.nxa, c0 = item / geo.nxa;
+    const double2* src = T + ((int64_t)c0 * geo.nxa + xi0) * geo.ps + line;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) nv[m] = nt_load(src + (j + 8 * m) * 64);
+  }
+  for (; item < nitems; item += gridDim.x) {
+    const int xi = item % geo.nxa, c = item / geo.nxa;
+    const int64_t st = state0 + c;
     double2 v[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = nt_load(src + (j + 8 * m) * 64);
-    double2* base = pl + line;
-    fft64_8x8<false>(v, base, kPL, j, stw);
+    for (int m = 0; m < 8; ++m) v[m] = nv[m];
+    const int nxt = item + gridDim.x;
+    if (nxt < nitems) {  // prefetch the next item's columns
+      const int xn = nxt % geo.nxa, cn = nxt / geo.nxa;
+      const double2* src = T + ((int64_t)cn * geo.nxa + xn) * geo.ps + line;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) base[(j + 8 * t) * kPL] = v[t];
-  }
-  __syncthreads();
-  const int r0 = geo.row_off[xi], nr = geo.row_off[xi + 1] - r0;
-  if (line < nr) {  // z transform of the mask rows
-    double2* base = pl + geo.rows[r0 + line] * kPL;
-    double2 v[8];
+      for (int m = 0; m < 8; ++m) nv[m] = nt_load(src + (j + 8 * m) * 64);
+    }
+    {  // y transform of column z = line
+      double2* base = pl + line;
+      fft64_8x8<false>(v, base, kPL, j, stw);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = base[j + 8 * m];
-    fft64_8x8<false>(v, base, 1, j, stw);
+      for (int t = 0; t < 8; ++t) base[(j + 8 * t) * kPL] = v[t];
+    }
+    __syncthreads();
+    const int r0 = geo.row_off[xi], nr = geo.row_off[xi + 1] - r0;
+    if (line < nr) {  // z transform of the mask rows
+      double2* base = pl + geo.rows[r0 + line] * kPL;
+      double2 w[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) base[j + 8 * t] = v[t];
+      for (int m = 0; m < 8; ++m) w[m] = base[j + 8 * m];
+      fft64_8x8<false>(w, base, 1, j, stw);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) base[j + 8 * t] = w[t];
+    }
+    __syncthreads();
+    const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
+    const int k = (int)((st / gp.nb) % gp.nk);
+    const double f = gp.occ[st];
+    const int mx = fftfreq_index(geo.plane_x[xi], 64);
+    const double ax = mx * gp.B[0] + gp.kpts[3 * k];
+    const double ay = mx * gp.B[1] + gp.kpts[3 * k + 1];
+    const double az = mx * gp.B[2] + gp.kpts[3 * k + 2];
+    const double2* qs = q + st * geo.ng;
+    double2* qb = qbar + st * geo.ng;
+    double kin = 0.0;
+    for (int g = g0 + tid; g < g1; g += kPlaneBlock) {
+      const int yz = geo.g_yz[g];
+      const int iy = yz >> 6, iz = yz & 63;
+      const int my = fftfreq_index(iy, 64), mz = fftfreq_index(iz, 64);
+      const double gx = ax + my * gp.B[3] + mz * gp.B[6];
+      const double gy = ay + my * gp.B[4] + mz * gp.B[7];
+      const double gz = az + my * gp.B[5] + mz * gp.B[8];
+      const double gk2 = gx * gx + gy * gy + gz * gz;
+      const double2 cv = qs[g];
+      const double2 hv = pl[iy * kPL + iz];
+      qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
+                           f * (gp.two_over_vol * hv.y + gk2 * cv.y));
+      kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
+    }
+    __syncthreads();
+    const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
+    if (tid == 0) kin_partial[st * geo.nxa + xi] = tot;
   }
-  __syncthreads();
-  const int g0 = geo.plane_g0[xi], g1 = geo.plane_g0[xi + 1];
-  const int k = (int)((st / gp.nb) % gp.nk);
-  const double f = gp.occ[st];
-  const int mx = fftfreq_index(geo.plane_x[xi], 64);
-  const double ax = mx * gp.B[0] + gp.kpts[3 * k];
-  const double ay = mx * gp.B[1] + gp.kpts[3 * k + 1];
-  const double az = mx * gp.B[2] + gp.kpts[3 * k + 2];
-  const double2* qs = q + st * geo.ng;
-  double2* qb = qbar + st * geo.ng;
-  double kin = 0.0;
-  for (int g = g0 + tid; g < g1; g += kPlaneBlock) {
-    const int yz = geo.g_yz[g];
-    const int iy = yz >> 6, iz = yz & 63;
-    const int my = fftfreq_index(iy, 64), mz = fftfreq_index(iz, 64);
-    const double gx = ax + my * gp.B[3] + mz * gp.B[6];
-    const double gy = ay + my * gp.B[4] + mz * gp.B[7];
-    const double gz = az + my * gp.B[5] + mz * gp.B[8];
-    const double gk2 = gx * gx + gy * gy + gz * gz;
-    const double2 cv = qs[g];
-    const double2 hv = pl[iy * kPL + iz];
-    qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
-                         f * (gp.two_over_vol * hv.y + gk2 * cv.y));
-    kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
-  }
-  __syncthreads();
-  const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
-  if (tid == 0) kin_partial[st * geo.nxa + xi] = tot;
 }
 
 // x-lines: block = 32 lines (fixed y, z0..z0+31), 8 lanes per line.
@@ -696,9 +718,11 @@ int launch_plane_fft_gather64(const double2* T, const double2* q,
                               const double2* tw64, const GatherParams& gp,
                               hipStream_t s) {
   size_t shm = (size_t)(64 * kPL + 64) * sizeof(double2);
-  hipLaunchKernelGGL(plane_fft_gather64_kernel, dim3(geo.nxa, nstates),
+  const int64_t nitems = (int64_t)geo.nxa * nstates;
+  const int64_t nblk = nitems < 1024 ? nitems : 1024;
+  hipLaunchKernelGGL(plane_fft_gather64_kernel, dim3((unsigned)nblk),
                      dim3(kPlaneBlock), shm, s, T, q, qbar, kin_partial,
-                     state0, geo, tw64, gp);
+                     state0, nstates, geo, tw64, gp);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
