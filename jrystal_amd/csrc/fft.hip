@@ -4,6 +4,15 @@
 
 namespace jrx {
 
+// Lengths with compile-time schedules (RadixOf): the x-line passes use
+// all of them; the plane passes use those whose n x n plane fits in LDS
+// (n <= 80), for cubic grids.
+#define JRX_CT_PLANE_SIZES(X)                                               \
+  X(32) X(35) X(36) X(40) X(42) X(45) X(48) X(49) X(50) X(54) X(56) X(60)   \
+  X(63) X(64) X(70) X(72) X(75) X(80)
+#define JRX_CT_SIZES(X) JRX_CT_PLANE_SIZES(X) X(128)
+
+
 // ------------------------------------------------------------ generic 3D --
 constexpr int kLineBlock = 256;
 constexpr int kLineCap = 2048;
@@ -145,15 +154,19 @@ int launch_plane_ifft(const double2* q, double2* T, int64_t state0,
   const int P = geo.ny * geo.nz;
   size_t shm = (size_t)P * sizeof(double2);
   dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
-  if (geo.ny == 64 && geo.nz == 64)
-    hipLaunchKernelGGL((plane_ifft_kernel<64, 64, 4096>), grid, block, shm, s,
-                       q, T, state0, geo, Ly, Lz);
-  else if (geo.ny == 32 && geo.nz == 32)
-    hipLaunchKernelGGL((plane_ifft_kernel<32, 32, 1024>), grid, block, shm, s,
-                       q, T, state0, geo, Ly, Lz);
-  else
-    hipLaunchKernelGGL((plane_ifft_kernel<0, 0, kPlaneCap>), grid, block, shm,
-                       s, q, T, state0, geo, Ly, Lz);
+  const int cube = geo.ny == geo.nz ? geo.ny : 0;
+#define JRX_PI(NN)                                                          \
+  case NN:                                                                  \
+    hipLaunchKernelGGL((plane_ifft_kernel<NN, NN, NN * NN>), grid, block,   \
+                       shm, s, q, T, state0, geo, Ly, Lz);                  \
+    break;
+  switch (cube) {
+    JRX_CT_PLANE_SIZES(JRX_PI)
+    default:
+      hipLaunchKernelGGL((plane_ifft_kernel<0, 0, kPlaneCap>), grid, block,
+                         shm, s, q, T, state0, geo, Ly, Lz);
+  }
+#undef JRX_PI
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -248,18 +261,20 @@ int launch_xline_density(const double2* T, double* rho_part, int groups,
   const int nzt = (geo.nz + LZ - 1) / LZ;
   size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
   dim3 grid(geo.ny * nzt, groups), block(kXBlock);
-  if (geo.nx == 64)
-    hipLaunchKernelGGL((xline_density_kernel<64>), grid, block, shm, s, T,
-                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
-  else if (geo.nx == 32)
-    hipLaunchKernelGGL((xline_density_kernel<32>), grid, block, shm, s, T,
-                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
-  else if (geo.nx == 128)
-    hipLaunchKernelGGL((xline_density_kernel<128>), grid, block, shm, s, T,
-                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
-  else
-    hipLaunchKernelGGL((xline_density_kernel<0>), grid, block, shm, s, T,
-                       rho_part, occ, state0, nstates, inv_vol, geo, Lx, LZ);
+#define JRX_XD(NN)                                                          \
+  case NN:                                                                  \
+    hipLaunchKernelGGL((xline_density_kernel<NN>), grid, block, shm, s, T,  \
+                       rho_part, occ, state0, nstates, inv_vol, geo, Lx,    \
+                       LZ);                                                 \
+    break;
+  switch (geo.nx) {
+    JRX_CT_SIZES(JRX_XD)
+    default:
+      hipLaunchKernelGGL((xline_density_kernel<0>), grid, block, shm, s, T,
+                         rho_part, occ, state0, nstates, inv_vol, geo, Lx,
+                         LZ);
+  }
+#undef JRX_XD
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -325,18 +340,18 @@ int launch_xline_vpsi(double2* T, const double* v, int groups, int nstates,
   const int nzt = (geo.nz + LZ - 1) / LZ;
   size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
   dim3 grid(geo.ny * nzt, groups), block(kXBlock);
-  if (geo.nx == 64)
-    hipLaunchKernelGGL((xline_vpsi_kernel<64>), grid, block, shm, s, T, v,
-                       nstates, geo, Lx, LZ);
-  else if (geo.nx == 32)
-    hipLaunchKernelGGL((xline_vpsi_kernel<32>), grid, block, shm, s, T, v,
-                       nstates, geo, Lx, LZ);
-  else if (geo.nx == 128)
-    hipLaunchKernelGGL((xline_vpsi_kernel<128>), grid, block, shm, s, T, v,
-                       nstates, geo, Lx, LZ);
-  else
-    hipLaunchKernelGGL((xline_vpsi_kernel<0>), grid, block, shm, s, T, v,
-                       nstates, geo, Lx, LZ);
+#define JRX_XV(NN)                                                          \
+  case NN:                                                                  \
+    hipLaunchKernelGGL((xline_vpsi_kernel<NN>), grid, block, shm, s, T, v,  \
+                       nstates, geo, Lx, LZ);                               \
+    break;
+  switch (geo.nx) {
+    JRX_CT_SIZES(JRX_XV)
+    default:
+      hipLaunchKernelGGL((xline_vpsi_kernel<0>), grid, block, shm, s, T, v,
+                         nstates, geo, Lx, LZ);
+  }
+#undef JRX_XV
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -421,18 +436,21 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
   size_t shm = (size_t)P * sizeof(double2);
   if (shm < kPlaneBlock * sizeof(double)) shm = kPlaneBlock * sizeof(double);
   dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
-  if (geo.ny == 64 && geo.nz == 64)
-    hipLaunchKernelGGL((plane_fft_gather_kernel<64, 64, 4096>), grid, block,
-                       shm, s, T, q, qbar, kin_partial, state0, geo, Ly, Lz,
-                       gp);
-  else if (geo.ny == 32 && geo.nz == 32)
-    hipLaunchKernelGGL((plane_fft_gather_kernel<32, 32, 1024>), grid, block,
-                       shm, s, T, q, qbar, kin_partial, state0, geo, Ly, Lz,
-                       gp);
-  else
-    hipLaunchKernelGGL((plane_fft_gather_kernel<0, 0, kPlaneCap>), grid,
-                       block, shm, s, T, q, qbar, kin_partial, state0, geo,
-                       Ly, Lz, gp);
+  const int cube = geo.ny == geo.nz ? geo.ny : 0;
+#define JRX_PG(NN)                                                          \
+  case NN:                                                                  \
+    hipLaunchKernelGGL((plane_fft_gather_kernel<NN, NN, NN * NN>), grid,    \
+                       block, shm, s, T, q, qbar, kin_partial, state0, geo, \
+                       Ly, Lz, gp);                                         \
+    break;
+  switch (cube) {
+    JRX_CT_PLANE_SIZES(JRX_PG)
+    default:
+      hipLaunchKernelGGL((plane_fft_gather_kernel<0, 0, kPlaneCap>), grid,
+                         block, shm, s, T, q, qbar, kin_partial, state0, geo,
+                         Ly, Lz, gp);
+  }
+#undef JRX_PG
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

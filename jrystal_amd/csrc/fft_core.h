@@ -212,9 +212,27 @@ template <int N>
 struct RadixOf {
   using type = void;  // no compile-time schedule: use lds_fft
 };
+// Same radix order as fft_factorize (8s, 4s, 2s, then 3, 5, 7), so the
+// compile-time and runtime schedules give identical results.
 template <> struct RadixOf<16> { using type = RadixList<8, 2>; };
 template <> struct RadixOf<32> { using type = RadixList<8, 4>; };
+template <> struct RadixOf<35> { using type = RadixList<5, 7>; };
+template <> struct RadixOf<36> { using type = RadixList<4, 3, 3>; };
+template <> struct RadixOf<40> { using type = RadixList<8, 5>; };
+template <> struct RadixOf<42> { using type = RadixList<2, 3, 7>; };
+template <> struct RadixOf<45> { using type = RadixList<3, 3, 5>; };
+template <> struct RadixOf<48> { using type = RadixList<8, 2, 3>; };
+template <> struct RadixOf<49> { using type = RadixList<7, 7>; };
+template <> struct RadixOf<50> { using type = RadixList<2, 5, 5>; };
+template <> struct RadixOf<54> { using type = RadixList<2, 3, 3, 3>; };
+template <> struct RadixOf<56> { using type = RadixList<8, 7>; };
+template <> struct RadixOf<60> { using type = RadixList<4, 3, 5>; };
+template <> struct RadixOf<63> { using type = RadixList<3, 3, 7>; };
 template <> struct RadixOf<64> { using type = RadixList<8, 8>; };
+template <> struct RadixOf<70> { using type = RadixList<2, 5, 7>; };
+template <> struct RadixOf<72> { using type = RadixList<8, 3, 3>; };
+template <> struct RadixOf<75> { using type = RadixList<3, 5, 5>; };
+template <> struct RadixOf<80> { using type = RadixList<8, 2, 5>; };
 template <> struct RadixOf<128> { using type = RadixList<8, 4, 4>; };
 
 template <bool INV, int BLOCK, int CAP, bool LINE_FAST, int N, int P, int R,

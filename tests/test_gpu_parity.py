@@ -191,3 +191,13 @@ def test_strained_gamma_only_si8():
   eo = plan.evaluate(params["w_re"], params["w_im"], grad=False)
   torch.cuda.synchronize()
   assert (full["energies"] - eo["energies"]).abs().max().item() < 1e-10
+
+
+@pytest.mark.parametrize("n", [36, 42, 45])
+def test_compile_time_radix_sizes(n):
+  """Cubic grids whose plane and x-line passes use the compile-time radix
+  schedules 4-3-3, 2-3-7 and 3-3-5."""
+  from jrystal_amd import workloads as W
+  wl = W.build(f"si2-{n}^3", W.fcc_si2(4.0), (n, n, n), (1, 1, 2), 8,
+               W.alias_free_cutoff(W.fcc_si2(4.0).cell_vectors, (n, n, n)))
+  _case(wl, seed=n)
