@@ -76,8 +76,14 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
     raise NotImplementedError(
       f"occupation {config.occupation!r}: only the fixed uniform / gamma "
       "occupations are on this path")
-  if config.optimizer != "adam":
-    raise NotImplementedError(f"optimizer {config.optimizer!r}")
+  if config.optimizer not in ("adam", "cg", "cg-learned"):
+    raise NotImplementedError(
+      f"optimizer {config.optimizer!r}: adam (the reference's), cg "
+      "(preconditioned conjugate gradients) or cg-learned (with the learned "
+      "preconditioner) are on this path")
+  if config.optimizer != "adam" and config.smearing > 0.0:
+    raise NotImplementedError("the cg optimisers minimise the energy at "
+                              "fixed occupations: set smearing to 0")
   if config.scheduler:
     raise NotImplementedError("Scheduler is not implemented yet.")
   cr = create_crystal(config)
@@ -100,6 +106,8 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   params = pw.param_init(config.seed, num_bands, nk, mask,
                          spin_restricted=config.spin_restricted,
                          dist=config.param_dist)
+  if config.optimizer != "adam":
+    return _calc_cg(config, cr, plan, params, occ, ew)
   opt_args = dict(config.optimizer_args)
   opt = Adam(params, **opt_args)
   if config.smearing > 0.0:
@@ -135,6 +143,12 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   seconds = time.perf_counter() - t0
   if not converged:
     log.warning("Did not converge.")
+  return _final(config, cr, plan, params, occ, ew, history, converged, i + 1,
+                seconds)
+
+
+def _final(config, cr, plan, params, occ, ew, history, converged, steps,
+           seconds):
   final = plan.evaluate(params["w_re"], params["w_im"], grad=False,
                         density=True)
   e = dict(zip(ENERGY_KEYS, final["energies"].tolist()))
@@ -148,8 +162,34 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   return GroundStateEnergyOutput(
     config=config, crystal=cr, params_pw=params, params_occ=occ,
     total_energy=e["total_with_nuclear"], total_energy_history=history,
-    energies=e, density=final["density"], converged=converged, steps=i + 1,
+    energies=e, density=final["density"], converged=converged, steps=steps,
     seconds=seconds)
+
+
+def _calc_cg(config, cr, plan, params, occ, ew):
+  """Preconditioned CG (optimizer: cg) or CG with the learned
+  preconditioner (optimizer: cg-learned) for at most `epoch` iterations,
+  stopping when the energy stops decreasing above rounding."""
+  from .. import learned
+  from ..minimize import Minimizer
+  factor_fn, restart = None, 60
+  if config.optimizer == "cg-learned":
+    factor_fn = learned.LearnedFactor(plan, learned.PrecondNet.load(
+      device=plan.device))
+    restart = learned.LEARNED_RESTART
+  m = Minimizer(plan, params, factor_fn=factor_fn, restart=restart)
+
+  def cb(mm, it, e):
+    if config.verbose and it % 20 == 0:
+      log.info("iteration %d | Energy %.8f", it, e + ew)
+  res = m.run(max_iter=config.epoch, callback=cb)
+  converged = res.status == "converged"
+  log.info("cg finished: %s after %d iterations (%d evaluations, %.2f s)",
+           res.status, res.iterations, res.evaluations, res.seconds)
+  history = [e for _, e in res.history]
+  p = {"w_re": m.params()["w_re"].clone(), "w_im": m.params()["w_im"].clone()}
+  return _final(config, cr, plan, p, occ, ew, history, converged,
+                res.iterations, res.seconds)
 
 
 def calc(config: JrystalConfigDict):

@@ -66,3 +66,22 @@ def test_cli_energy_run(tmp_path):
   # uniform occupation: 2/nk on ceil(28/2) = 14 of 16 bands, nk = 1
   np.testing.assert_allclose(rho.sum() * vol / rho[0].size, 28.0, rtol=1e-12)
   assert out.total_energy_history[-1] < out.total_energy_history[0]
+
+
+@pytest.mark.parametrize("opt", ["cg", "cg-learned"])
+def test_cli_cg_optimisers_reach_lower_energy_than_adam(tmp_path, opt):
+  from jrystal_amd import config
+  from jrystal_amd.calc.energy_all_electrons import calc_with_output
+  base = ("crystal: si\ngrid_sizes: 16\nk_grid_sizes: 1\ncutoff_energy: 10\n"
+          "empty_bands: 2\nverbose: false\nseed: 1\nsmearing: 0.0\n")
+  pa, pc = tmp_path / "a.yaml", tmp_path / "c.yaml"
+  pa.write_text(base + "epoch: 300\n")
+  pc.write_text(base + f"epoch: 300\noptimizer: {opt}\n")
+  adam = calc_with_output(config.get_config(str(pa)))
+  cg = calc_with_output(config.get_config(str(pc)))
+  e = cg.total_energy_history
+  assert all(b <= a + 1e-12 for a, b in zip(e, e[1:]))
+  assert cg.total_energy < adam.total_energy
+  rho = cg.density.cpu().numpy()
+  np.testing.assert_allclose(rho.sum() * cg.crystal.vol / rho[0].size, 28.0,
+                             rtol=1e-12)
