@@ -118,6 +118,9 @@ SIGNATURES = {
   "jrx_mlp_mse_workspace_bytes": [_I64],
   "jrx_mlp_mse_loss": [_P, _P, _I64, _P, _P, _P, _SZ, _P],
   "jrx_mlp_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I64, _P],
+  "jrx_cg_dots_workspace_bytes": [],
+  "jrx_plan_precondition_dots": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                 _P, _SZ, _P, _SZ, _P],
   "jrx_dot_workspace_bytes": [],
   "jrx_vec_dot": [_P, _P, _I64, _P, _P, _SZ, _P],
   "jrx_vec_axpby": [_D, _P, _D, _P, _P, _I64, _P],
@@ -138,6 +141,7 @@ _RESTYPES = {"jrx_last_error": ctypes.c_char_p,
              "jrx_reduce_workspace_bytes": ctypes.c_size_t,
              "jrx_ewald_workspace_bytes": ctypes.c_size_t,
              "jrx_dot_workspace_bytes": ctypes.c_size_t,
+             "jrx_cg_dots_workspace_bytes": ctypes.c_size_t,
              "jrx_mlp_num_params": ctypes.c_size_t,
              "jrx_mlp_backward_workspace_bytes": ctypes.c_size_t,
              "jrx_mlp_mse_workspace_bytes": ctypes.c_size_t}

@@ -475,6 +475,109 @@ extern "C" int jrx_plan_precondition(jrx_plan* p, const double* g_re,
   return JRX_OK;
 }
 
+// Fused CG step quantities: h1 = K(|G+k|^2/tb) * factor * g1 over the
+// parameters (real and imaginary planes), and block partial sums of
+// g1.h1, g0.h1 and g1.d.  Block b takes the contiguous slice
+// [b*chunk, (b+1)*chunk) of each plane; fixed-order sums.
+constexpr int kFuseBlocks = 1024;
+__global__ void __launch_bounds__(256)
+tpa_dots_kernel(const double* __restrict__ g1r, const double* __restrict__ g1i,
+                const double* __restrict__ g0r, const double* __restrict__ g0i,
+                const double* __restrict__ dr, const double* __restrict__ di,
+                const double* __restrict__ tb, const short* __restrict__ g_m,
+                const double* __restrict__ kpts, CellParams cp, int ng,
+                int nk, int nb, int64_t n, int64_t chunk,
+                const double* __restrict__ factor, double* __restrict__ hr,
+                double* __restrict__ hi, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi_end = lo + chunk < n ? lo + chunk : n;
+  double s11 = 0.0, s01 = 0.0, s1d = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi_end; i += 256) {
+    const int b = (int)(i % nb);
+    const int64_t rest = i / nb;
+    const int g = (int)(rest % ng);
+    const int64_t sk = rest / ng;
+    const int k = (int)(sk % nk);
+    const double t = tb[sk * nb + b];
+    const double x = coef_gk2(g_m, kpts, cp, g, k) / (t > 1e-300 ? t : 1e-300);
+    const double pp = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+    double kx = pp / (pp + 16.0 * x * x * x * x);
+    if (factor) kx *= factor[i];
+    const double ar = g1r[i], ai = g1i[i];
+    const double h_r = kx * ar, h_i = kx * ai;
+    hr[i] = h_r;
+    hi[i] = h_i;
+    s11 += ar * h_r + ai * h_i;
+    s01 += g0r[i] * h_r + g0i[i] * h_i;
+    s1d += ar * dr[i] + ai * di[i];
+  }
+  s11 = block_sum(s11, red);
+  s01 = block_sum(s01, red);
+  s1d = block_sum(s1d, red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s11;
+    part[3 * blockIdx.x + 1] = s01;
+    part[3 * blockIdx.x + 2] = s1d;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+sum3_kernel(const double* __restrict__ part, int nblk,
+            double* __restrict__ out) {
+  __shared__ double red[256];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c += part[3 * i + 2];
+  }
+  a = block_sum(a, red);
+  b = block_sum(b, red);
+  c = block_sum(c, red);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+  }
+}
+
+extern "C" size_t jrx_cg_dots_workspace_bytes(void) {
+  return (size_t)(3 * kFuseBlocks + 8) * sizeof(double);
+}
+
+extern "C" int jrx_plan_precondition_dots(
+    jrx_plan* p, const double* g1_re, const double* g1_im,
+    const double* g0_re, const double* g0_im, const double* d_re,
+    const double* d_im, const double* factor, double* h1_re, double* h1_im,
+    double* out, void* scratch, size_t scratch_bytes, const void* workspace,
+    size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && g1_re && g1_im && g0_re && g0_im && d_re && d_im &&
+                  h1_re && h1_im && out && scratch && workspace,
+              "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  JRX_REQUIRE(scratch_bytes >= jrx_cg_dots_workspace_bytes(),
+              "scratch too small (jrx_cg_dots_workspace_bytes)");
+  const int64_t n = p->S * p->ng;
+  int64_t nblk = (n + 256 * 16 - 1) / (256 * 16);
+  if (nblk < 1) nblk = 1;
+  if (nblk > kFuseBlocks) nblk = kFuseBlocks;
+  const int64_t chunk = (n + nblk - 1) / nblk;
+  const double* tb = reinterpret_cast<const double*>(
+      static_cast<const char*>(workspace) + p->o_tb);
+  double* part = static_cast<double*>(scratch);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(tpa_dots_kernel, dim3((unsigned)nblk), dim3(256), 0, s,
+                     g1_re, g1_im, g0_re, g0_im, d_re, d_im, tb, p->d_g_m,
+                     p->d_kpts, p->cp, p->ng, p->nk, p->nb, n, chunk, factor,
+                     h1_re, h1_im, part);
+  JRX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, s, part, (int)nblk,
+                     out);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
 extern "C" int jrx_plan_band_norms(const jrx_plan* p, double* out,
                                    const void* workspace,
                                    size_t workspace_bytes, void* stream) {

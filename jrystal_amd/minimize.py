@@ -76,9 +76,11 @@ class Minimizer:
     self.e = torch.empty(5, dtype=torch.float64, device=dev)
     self.e1 = torch.empty(5, dtype=torch.float64, device=dev)
     self.dotv = torch.empty(4, dtype=torch.float64, device=dev)
-    ws = _lib.lib().jrx_dot_workspace_bytes()
+    ws = max(_lib.lib().jrx_dot_workspace_bytes(),
+             _lib.lib().jrx_cg_dots_workspace_bytes())
     self._dws = torch.empty(ws + 256, dtype=torch.uint8, device=dev)
     self._dws_ptr = (self._dws.data_ptr() + 255) // 256 * 256
+    self._s0_next = None  # g.d of the next iteration, from the fused pass
     self.factor_fn = factor_fn
     self.restarts = 0       # gauge resets so far
     self._plain_cycles = 0  # restart cycles left on the plain preconditioner
@@ -103,6 +105,20 @@ class Minimizer:
     if self.factor_fn is not None and self._plain_cycles == 0:
       factor = self.factor_fn(self)
     self.plan.precondition(g[0], g[1], factor, h[0], h[1])
+
+  def _precondition_dots(self):
+    """h1 = P g1 and dotv[0..2] = (g1.h1, g.h1, g1.d) in one pass."""
+    factor = None
+    if self.factor_fn is not None and self._plain_cycles == 0:
+      factor = self.factor_fn(self)
+    p = self.plan
+    _lib.check(_lib.lib().jrx_plan_precondition_dots(
+      p._plan, _lib.ptr(self.g1[0]), _lib.ptr(self.g1[1]),
+      _lib.ptr(self.g[0]), _lib.ptr(self.g[1]), _lib.ptr(self.d[0]),
+      _lib.ptr(self.d[1]), _lib.ptr(factor), _lib.ptr(self.h1[0]),
+      _lib.ptr(self.h1[1]), _lib.ptr(self.dotv), self._dws_ptr,
+      self._dws.numel() - 256, p._ws_ptr, p.workspace_bytes,
+      _lib.stream_ptr(p.device)))
 
   def _dot(self, a, b, slot=0):
     _lib.check(_lib.lib().jrx_vec_dot(
@@ -146,6 +162,7 @@ class Minimizer:
     gradient at the new W, and the preconditioned steepest-descent
     direction.  The last evaluation must carry a gradient.  Returns
     (E, g.Pg)."""
+    self._s0_next = None
     if self.ritz:
       self.plan.rotate(self._ritz_rotation(), self.W[0], self.W[1])
     else:
@@ -208,8 +225,11 @@ class Minimizer:
     since_restart = 0
     it = 0
     for it in range(1, max_iter + 1):
-      self._dot(self.g, self.d, 0)
-      s0 = float(self.dotv[0])
+      if self._s0_next is not None:
+        s0, self._s0_next = self._s0_next, None
+      else:
+        self._dot(self.g, self.d, 0)
+        s0 = float(self.dotv[0])
       if not (np.isfinite(s0) and np.isfinite(gh)):
         # a preconditioner factor that is not finite: plain preconditioner
         fallbacks += 1
@@ -288,12 +308,13 @@ class Minimizer:
         E1, gh = self._restart()
         since_restart = 0
       else:
-        self._precondition(self.g1, self.h1)
-        self._dot(self.g1, self.h1, 0)
-        self._dot(self.g, self.h1, 1)
-        g1h1, g0h1 = float(self.dotv[0]), float(self.dotv[1])
+        # h1 = P g1 and the dots of the PR+ update in one pass; the next
+        # slope g1.d_new = -g1.h1 + beta g1.d needs no further pass
+        self._precondition_dots()
+        g1h1, g0h1, g1d = (float(x) for x in self.dotv[:3].tolist())
         beta = max(0.0, (g1h1 - g0h1) / gh) if gh > 0 else 0.0
         self._axpby(-1.0, self.h1, beta, self.d, self.d)
+        self._s0_next = -g1h1 + beta * g1d
         self.g, self.g1 = self.g1, self.g
         self.h, self.h1 = self.h1, self.h
         gh = g1h1

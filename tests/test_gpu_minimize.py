@@ -42,6 +42,51 @@ def test_band_norms_and_tpa_match_oracle():
     assert rel_max(hi.cpu().numpy(), ref["w_im"]) < 1e-12
 
 
+def test_fused_precondition_dots_matches_oracle():
+  """jrx_plan_precondition_dots: h1 = K factor g1 as the oracle's TPA step,
+  and (g1.h1, g0.h1, g1.d) as numpy dots of the same arrays."""
+  from jrystal_amd import _lib
+  from jrystal_amd.objective import EnergyPlan
+  wl, prob = _si2()
+  p = wl.params(3)
+  plan = EnergyPlan.from_workload(wl)
+  plan.evaluate(torch.tensor(p["w_re"], device="cuda"),
+                torch.tensor(p["w_im"], device="cuda"), grad=True)
+  L = _lib.lib()
+  rng = np.random.default_rng(17)
+  shp = plan.param_shape
+  g1, g0, d = (rng.standard_normal((2,) + shp) for _ in range(3))
+  f = rng.uniform(0.5, 2.0, shp)
+  ws = L.jrx_cg_dots_workspace_bytes()
+  scratch = torch.empty(ws, dtype=torch.uint8, device="cuda")
+  t = lambda a: torch.tensor(a, device="cuda")  # noqa
+  g1t, g0t, dt = t(g1), t(g0), t(d)
+  for factor in (None, f):
+    ft = None if factor is None else t(factor)
+    h = torch.full_like(g1t, np.nan)
+    out = torch.full((4,), np.nan, dtype=torch.float64, device="cuda")
+    _lib.check(L.jrx_plan_precondition_dots(
+      plan._plan, g1t[0].data_ptr(), g1t[1].data_ptr(), g0t[0].data_ptr(),
+      g0t[1].data_ptr(), dt[0].data_ptr(), dt[1].data_ptr(), _lib.ptr(ft),
+      h[0].data_ptr(), h[1].data_ptr(), out.data_ptr(), scratch.data_ptr(),
+      ws, plan._ws_ptr, plan.workspace_bytes, _lib.stream_ptr()))
+    ref = OOpt.tpa_precondition({"w_re": g1[0], "w_im": g1[1]}, p, prob,
+                                factor)
+    href = np.stack([ref["w_re"], ref["w_im"]])
+    hh = h.cpu().numpy()
+    assert rel_max(hh, href) < 1e-12
+    o = out.cpu().numpy()
+    for got, want in ((o[0], np.sum(g1 * href)), (o[1], np.sum(g0 * href)),
+                      (o[2], np.sum(g1 * d))):
+      assert abs(got - want) <= 1e-12 * np.sum(np.abs(g1) * np.abs(d) + 1.0)
+  with pytest.raises(ValueError):
+    _lib.check(L.jrx_plan_precondition_dots(
+      plan._plan, g1t[0].data_ptr(), g1t[1].data_ptr(), g0t[0].data_ptr(),
+      g0t[1].data_ptr(), dt[0].data_ptr(), dt[1].data_ptr(), None,
+      h[0].data_ptr(), h[1].data_ptr(), out.data_ptr(), scratch.data_ptr(),
+      ws - 8, plan._ws_ptr, plan.workspace_bytes, _lib.stream_ptr()))
+
+
 def test_energy_only_evaluation_sets_band_norms():
   from jrystal_amd.objective import EnergyPlan
   wl, prob = _si2()
