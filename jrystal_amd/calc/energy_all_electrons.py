@@ -171,8 +171,8 @@ def _calc_cg(config, cr, plan, params, occ, ew):
   preconditioner (optimizer: cg-learned) for at most `epoch` iterations,
   stopping when the energy stops decreasing above rounding."""
   from .. import learned
-  from ..minimize import Minimizer
-  factor_fn, restart = None, 60
+  from ..minimize import PLAIN_RESTART, Minimizer
+  factor_fn, restart = None, PLAIN_RESTART
   if config.optimizer == "cg-learned":
     factor_fn = learned.LearnedFactor(plan, learned.PrecondNet.load(
       device=plan.device))

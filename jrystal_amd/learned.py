@@ -43,9 +43,10 @@ from .objective import EnergyPlan
 NFEAT = 8
 # CG restart interval (iterations) used with the learned factor, which is
 # refreshed at every restart.  Chosen on training-distribution cells
-# (strained 8-atom cubes, strain seeds 11-13, 2x2x2 k): 30 beat 60 and 120;
-# the plain method's best interval on the same cells is the default 60.
-LEARNED_RESTART = 30
+# (strained 8-atom cubes, strain seeds 11-13, 4x4x4 k), with one plain first
+# cycle: 45 beat 30 and 60 (time to |dE| < 1e-6 Ha); the plain method's best
+# interval on the same cells is minimize.PLAIN_RESTART.
+LEARNED_RESTART = 45
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                              "data", "precond_mlp.npz")
 
@@ -181,8 +182,11 @@ class LearnedFactor:
     self._stamp = None
 
   def __call__(self, minimizer):
-    if minimizer.restarts < self.first_cycle:
-      return None  # plain TPA while the iterate is far from the minimum
+    # restarts counts the gauge reset that starts a run, so cycle c runs
+    # with restarts == c; the first first_cycle cycles use plain TPA, while
+    # the iterate is still far from the states the network was trained on
+    if minimizer.restarts <= self.first_cycle:
+      return None
     if minimizer.restarts != self._stamp:
       density_stats(self.plan, self.stats)
       gap = band_gaps(self.plan) if self.net.use_gap else None

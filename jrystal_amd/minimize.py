@@ -23,7 +23,14 @@ classic plane-wave alternative: preconditioned conjugate gradients on W
 * acceptance only when the energy does not increase; otherwise up to
   ``backtracks`` parabolic backtracking steps, and if those fail too, a
   fall-back to the plain preconditioner for two restart cycles from a
-  steepest-descent restart.
+  steepest-descent restart;
+* a stall check: ``flat_window`` iterations without a decrease above
+  rounding while the first-order decrease of a step, g.Pg times the step
+  (invariant under the drift of |W| between gauge resets), is still far
+  from zero (the steps have collapsed, conjugacy is lost) trigger a
+  restart with the initial step size; the run stops as "converged" only
+  when the energy is flat and that decrease is small, and as "stalled"
+  after ``max_stalls`` stall restarts in a row.
 
 Every vector operation is a HIP kernel (jrx_vec_dot, jrx_vec_axpby,
 jrx_plan_precondition, jrx_plan_rotate); the host sees a handful of
@@ -41,6 +48,11 @@ from . import _lib
 from .objective import EnergyPlan
 
 
+# restart interval of plain TPA CG: the fastest to |dE| < 1e-6 Ha of 60, 120
+# and no periodic restart on three strained 8-atom cells (4x4x4 k-points)
+PLAIN_RESTART = 120
+
+
 @dataclass
 class MinimizeResult:
   energy: float
@@ -56,7 +68,7 @@ class MinimizeResult:
 class Minimizer:
 
   def __init__(self, plan: EnergyPlan, params, factor_fn: Optional[Callable]
-               = None, restart: int = 60, step0: float = 0.5,
+               = None, restart: int = PLAIN_RESTART, step0: float = 0.5,
                line_search: str = "parabola", ritz: bool = True):
     if line_search not in ("secant", "parabola"):
       raise ValueError(f"unknown line search {line_search!r}")
@@ -85,7 +97,10 @@ class Minimizer:
     self.restarts = 0       # gauge resets so far
     self._plain_cycles = 0  # restart cycles left on the plain preconditioner
     self.restart = restart
+    self.step0 = step0
     self.step = step0
+    self.max_stalls = 3
+    self.stall_slope = 1e-10  # g.Pg step / max(1, |E|) above which flat = stall
     self.backtracks = 6
     self.flat_window = 12
     self.flat_tol = 1e-14
@@ -156,12 +171,17 @@ class Minimizer:
         b = e + 1
     return torch.from_numpy(V)
 
-  def _restart(self):
+  def _restart(self, gh_before=None):
     """Gauge reset W <- Q V of the last evaluation (same energy; R = I and,
     with ``ritz``, Rayleigh-Ritz bands within each occupation block), the
     gradient at the new W, and the preconditioned steepest-descent
     direction.  The last evaluation must carry a gradient.  Returns
-    (E, g.Pg)."""
+    (E, g.Pg).
+
+    The reset undoes the drift of |W| since the last one; g.Pg scales as
+    1/|W|^2 and the step as |W|^2, so with ``gh_before`` (g.Pg before the
+    reset) the step is rescaled by gh_before / g.Pg and the next trial
+    step starts from it."""
     self._s0_next = None
     if self.ritz:
       self.plan.rotate(self._ritz_rotation(), self.W[0], self.W[1])
@@ -172,7 +192,13 @@ class Minimizer:
     self._precondition(self.g, self.h)
     self._axpby(-1.0, self.h, 0.0, self.h, self.d)
     self._dot(self.g, self.h, 0)
-    return float(self.e[4]), float(self.dotv[0])
+    gh = float(self.dotv[0])
+    if gh_before is not None:
+      ratio = gh_before / gh if gh > 0 else np.nan
+      if np.isfinite(ratio) and ratio > 0:
+        self.step *= ratio
+        self._s0_prev = None
+    return float(self.e[4]), gh
 
   # -- line searches along d from W (energy E0, slope s0 = g.d < 0) --------
   def _search_parabola(self, E0, s0, lt):
@@ -199,8 +225,8 @@ class Minimizer:
     Et = float(self.e1[4])
     self._dot(self.g1, self.d, 2)
     s1 = float(self.dotv[2])
-    if np.isfinite(Et) and s0 - s1 > 0:
-      lam = lt * s0 / (s0 - s1)
+    if np.isfinite(Et) and np.isfinite(s1) and s1 > s0:
+      lam = lt * s0 / (s0 - s1)  # root of the slope, linear in lam
     else:
       c = (Et - E0 - s0 * lt) / (lt * lt) if np.isfinite(Et) else 1.0
       lam = -s0 / (2.0 * c) if c > 0 else 4.0 * lt
@@ -220,9 +246,10 @@ class Minimizer:
     self._eval(self.W, self.e, self.g)  # Q of the starting point
     E0, gh = self._restart()
     hist = [(time.perf_counter() - t0, E0)]
-    rejected = fallbacks = stalled = 0
+    rejected = fallbacks = stalled = stalls = 0
     status = "max_iter"
     since_restart = 0
+    window_start = 0  # the flat-energy window looks at hist[window_start:]
     it = 0
     for it in range(1, max_iter + 1):
       if self._s0_next is not None:
@@ -248,8 +275,11 @@ class Minimizer:
         self._axpby(-1.0, self.h, 0.0, self.h, self.d)
         s0 = -gh
         since_restart = 0
-      if -s0 <= self.slope_tol * max(1.0, abs(E0)):
-        status = "converged"  # slope at the rounding level of E
+      # E(W) = E(W T): between gauge resets |W| drifts, g.Pg scales as
+      # 1/|W|^2 and the step as |W|^2; their product, the first-order
+      # decrease of a step, does not
+      if gh * self.step <= self.slope_tol * max(1.0, abs(E0)):
+        status = "converged"  # decrease per step at the rounding of E
         break
       # initial trial step: keep the first-order decrease lam * s0 of the
       # last accepted step (Nocedal & Wright, Numerical Optimization, 3.5)
@@ -290,7 +320,7 @@ class Minimizer:
           self._s0_prev = None
           self._plain_cycles = 2
           self._eval(self.W, self.e, self.g)
-          E0, gh = self._restart()
+          E0, gh = self._restart(gh_before=gh)
           since_restart = 0
           hist.append((time.perf_counter() - t0, E0))
           if stalled >= 3:
@@ -305,7 +335,7 @@ class Minimizer:
       since_restart += 1
       if since_restart >= self.restart:
         self._plain_cycles = max(0, self._plain_cycles - 1)
-        E1, gh = self._restart()
+        E1, gh = self._restart(gh_before=gh)
         since_restart = 0
       else:
         # h1 = P g1 and the dots of the PR+ update in one pass; the next
@@ -325,10 +355,29 @@ class Minimizer:
       if e_stop is not None and E0 <= e_stop:
         status = "e_stop"
         break
-      if (len(hist) > self.flat_window and hist[-self.flat_window - 1][1] - E0
-          <= self.flat_tol * max(1.0, abs(E0))):
-        status = "converged"  # no decrease above rounding for a while
-        break
+      if len(hist) - window_start > self.flat_window:
+        if (hist[-self.flat_window - 1][1] - E0
+            > self.flat_tol * max(1.0, abs(E0))):
+          stalls = 0
+        else:
+          # no decrease above rounding for flat_window iterations: a
+          # minimum when the first-order decrease per step is small too,
+          # else a stall
+          if gh * self.step <= self.stall_slope * max(1.0, abs(E0)):
+            status = "converged"
+            break
+          stalls += 1
+          if stalls > self.max_stalls:
+            status = "stalled"
+            break
+          # the steps collapsed with the slope far from zero: restart
+          # with the initial step size
+          self.step = self.step0
+          self._s0_prev = None
+          E0, gh = self._restart()
+          since_restart = 0
+          window_start = len(hist)
+          hist.append((time.perf_counter() - t0, E0))
       if time_limit is not None and hist[-1][0] > time_limit:
         status = "time_limit"
         break

@@ -232,6 +232,33 @@ def test_trained_network_gains_over_plain_preconditioner():
   assert all(b <= a + 1e-12 for a, b in zip(e, e[1:]))
 
 
+@pytest.mark.parametrize("mode", ["learned_secant", "plain_no_restart"])
+def test_other_schedules_reach_the_minimum(mode):
+  """The secant line search with the learned factor (its slope at the trial
+  point is often just below the slope at 0, where the secant has no
+  positive root) and CG without periodic restarts (stalls restart it) end
+  at the parabola-search minimum."""
+  from jrystal_amd import learned
+  from jrystal_amd.minimize import Minimizer
+  from jrystal_amd.objective import EnergyPlan
+  wl = _strained_cell()
+  plan = EnergyPlan.from_workload(wl)
+  ref = Minimizer(plan, wl.params(0)).run(max_iter=2000)
+  if mode == "learned_secant":
+    m = Minimizer(plan, wl.params(0), line_search="secant",
+                  factor_fn=learned.LearnedFactor(plan,
+                                                  learned.PrecondNet.load()),
+                  restart=learned.LEARNED_RESTART)
+  else:
+    m = Minimizer(plan, wl.params(0), restart=10**9)
+  r = m.run(max_iter=3000)
+  print(mode, r.iterations, r.status, r.energy - ref.energy)
+  assert r.status in ("converged", "stalled")
+  assert abs(r.energy - ref.energy) < 1e-7
+  e = [h[1] for h in r.history]
+  assert all(b <= a + 1e-12 for a, b in zip(e, e[1:]))
+
+
 def test_fallback_to_plain_preconditioner():
   """A factor that breaks the search direction (NaN) sends the minimiser
   back to the plain preconditioner; it still reaches the plain minimum with
