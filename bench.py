@@ -151,8 +151,11 @@ def convergence(time_cap, tol=1e-6):
     if mode == "learned_warm":
       p0 = learned.warm_start(plan, net, p0)
     fn = None if mode == "plain" else learned.LearnedFactor(plan, net)
-    m = Minimizer(plan, p0, factor_fn=fn, restart=(
-      PLAIN_RESTART if mode == "plain" else learned.LEARNED_RESTART))
+    if mode == "plain":
+      m = Minimizer(plan, p0, restart=PLAIN_RESTART)
+    else:
+      m = Minimizer(plan, p0, factor_fn=fn, restart=learned.LEARNED_RESTART,
+                    first_restart=learned.LEARNED_FIRST_RESTART)
     t_setup = time.perf_counter() - t0
     stop = None if e_best is None else e_best + 0.1 * tol
     r = m.run(max_iter=100000, time_limit=time_cap, e_stop=stop)
@@ -164,7 +167,8 @@ def convergence(time_cap, tol=1e-6):
          "time_cap_s": time_cap,
          "schedule": {"plain_restart": PLAIN_RESTART,
                       "learned_restart": learned.LEARNED_RESTART,
-                      "learned_plain_first_cycles": 1,
+                      "learned_plain_first_cycle": (
+                        learned.LEARNED_FIRST_RESTART),
                       "line_search": "parabola"}}
   for mode, r in runs.items():
     t, i = time_to(r.history, e_star, tol)

@@ -177,7 +177,9 @@ def _calc_cg(config, cr, plan, params, occ, ew):
     factor_fn = learned.LearnedFactor(plan, learned.PrecondNet.load(
       device=plan.device))
     restart = learned.LEARNED_RESTART
-  m = Minimizer(plan, params, factor_fn=factor_fn, restart=restart)
+  m = Minimizer(plan, params, factor_fn=factor_fn, restart=restart,
+                first_restart=(learned.LEARNED_FIRST_RESTART
+                               if factor_fn is not None else None))
 
   def cb(mm, it, e):
     if config.verbose and it % 20 == 0:

@@ -47,6 +47,9 @@ NFEAT = 8
 # cycle: 45 beat 30 and 60 (time to |dE| < 1e-6 Ha); the plain method's best
 # interval on the same cells is minimize.PLAIN_RESTART.
 LEARNED_RESTART = 45
+# length of the plain first cycle before the factor is used (same cells:
+# 30 beat 10, 20, 45 and 60)
+LEARNED_FIRST_RESTART = 30
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                              "data", "precond_mlp.npz")
 
@@ -496,7 +499,9 @@ def select_by_convergence(nets, cells, tol=1e-6, max_iter=3000, log=None):
     runs = []
     for n, net in enumerate(nets):
       runs.append(Minimizer(plan, p0, factor_fn=LearnedFactor(plan, net),
-                            restart=LEARNED_RESTART).run(max_iter=max_iter))
+                            restart=LEARNED_RESTART,
+                            first_restart=LEARNED_FIRST_RESTART)
+                  .run(max_iter=max_iter))
       if log:
         log(f"{wl.name}: candidate {n}: {runs[-1].iterations} iterations")
     e_star = min([plain.energy] + [r.energy for r in runs])

@@ -69,7 +69,8 @@ class Minimizer:
 
   def __init__(self, plan: EnergyPlan, params, factor_fn: Optional[Callable]
                = None, restart: int = PLAIN_RESTART, step0: float = 0.5,
-               line_search: str = "parabola", ritz: bool = True):
+               line_search: str = "parabola", ritz: bool = True,
+               first_restart: Optional[int] = None):
     if line_search not in ("secant", "parabola"):
       raise ValueError(f"unknown line search {line_search!r}")
     self.line_search = line_search
@@ -97,6 +98,8 @@ class Minimizer:
     self.restarts = 0       # gauge resets so far
     self._plain_cycles = 0  # restart cycles left on the plain preconditioner
     self.restart = restart
+    # length of the first restart cycle (default: restart)
+    self.first_restart = restart if first_restart is None else first_restart
     self.step0 = step0
     self.step = step0
     self.max_stalls = 3
@@ -333,7 +336,8 @@ class Minimizer:
       self.W, self.W1 = self.W1, self.W
       self.e, self.e1 = self.e1, self.e
       since_restart += 1
-      if since_restart >= self.restart:
+      if since_restart >= (self.first_restart if self.restarts == 1
+                           else self.restart):
         self._plain_cycles = max(0, self._plain_cycles - 1)
         E1, gh = self._restart(gh_before=gh)
         since_restart = 0

@@ -220,7 +220,8 @@ def test_trained_network_gains_over_plain_preconditioner():
   net = learned.PrecondNet.load()
   plain = Minimizer(plan, wl.params(0)).run(max_iter=2000)
   lrn = Minimizer(plan, wl.params(0), factor_fn=learned.LearnedFactor(plan, net),
-                  restart=learned.LEARNED_RESTART).run(max_iter=2000)
+                  restart=learned.LEARNED_RESTART,
+                  first_restart=learned.LEARNED_FIRST_RESTART).run(max_iter=2000)
   e_star = min(plain.energy, lrn.energy)
   assert abs(plain.energy - lrn.energy) < 1e-7
 
