@@ -62,18 +62,36 @@ __host__ __device__ inline double2 cconj(double2 a) {
 }
 __host__ __device__ inline double2 czero() { return make_double2(0.0, 0.0); }
 
+// Sum over the 64 lanes of a wavefront (xor butterfly: the same order on
+// every run); every lane receives the sum.  No LDS and no block barrier.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Sum of the values each thread of a block stored in sh[0, blockDim.x),
+// after the barrier that follows those stores; wave 0 reduces (in a fixed
+// order) and lane 0 of wave 0 gets the sum.  Needs only the caller's one
+// barrier.
+__device__ inline double wave0_sum(const double* sh) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < (int)blockDim.x; i += 64) a += sh[i];
+  return wave_sum(a);
+}
+
 // Fixed-order block reduction of one double (blockDim.x a multiple of 64,
-// at most 1024).  Deterministic: the tree shape depends only on blockDim.
+// at most 1024; sh holds blockDim.x / 64 doubles).  Deterministic: the
+// order depends only on blockDim.  Every thread receives the sum.
 __device__ inline double block_sum(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
+  // wave partials by shuffles, then every thread sums them in wave order
+  v = wave_sum(v);
+  const int nw = ((int)blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
-    __syncthreads();
-  }
-  double r = sh[0];
-  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < nw; ++w) r += sh[w];
+  __syncthreads();  // sh may be reused right after
   return r;
 }
 

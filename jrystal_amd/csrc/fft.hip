@@ -423,8 +423,13 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
     kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
   }
   __syncthreads();  // the plane is dead: reuse its LDS for the reduction
-  const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
-  if (threadIdx.x == 0) kin_partial[st * geo.nxa + xi] = tot;
+  double* red = reinterpret_cast<double*>(sm);
+  red[threadIdx.x] = kin;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const double tot = wave0_sum(red);
+    if (threadIdx.x == 0) kin_partial[st * geo.nxa + xi] = tot;
+  }
 }
 
 int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
@@ -526,7 +531,9 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_ifft64_kernel(
 // buffer while it transforms and gathers the current one (only two blocks
 // fit a CU's LDS, so in-block prefetch is what hides the load latency;
 // measured 7% faster than one block per item).
-__global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather64_kernel(
+// Two blocks fit a CU's LDS (72.7 KB each); both are resident only at 4
+// waves per SIMD, i.e. at most 128 VGPRs, hence the second bound.
+__global__ __launch_bounds__(kPlaneBlock, 4) void plane_fft_gather64_kernel(
     const double2* __restrict__ T, const double2* __restrict__ q,
     double2* __restrict__ qbar, double* __restrict__ kin_partial,
     int64_t state0, int nstates, BoxGeom geo,
@@ -534,6 +541,7 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather64_kernel(
   extern __shared__ double2 sm[];
   double2* pl = sm;
   double2* stw = sm + 64 * kPL;
+  double* red = reinterpret_cast<double*>(sm + 64 * kPL + 64);  // [512]
   const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
   const int nitems = geo.nxa * nstates;
   if (tid < 64) stw[tid] = tw64[tid];
@@ -601,9 +609,14 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather64_kernel(
                            f * (gp.two_over_vol * hv.y + gk2 * cv.y));
       kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
     }
-    __syncthreads();
-    const double tot = block_sum(kin, reinterpret_cast<double*>(sm));
-    if (tid == 0) kin_partial[st * geo.nxa + xi] = tot;
+    // red lies outside pl; wave 0 reads it before its next barrier, and no
+    // thread stores to it again before passing that barrier
+    red[tid] = kin;
+    __syncthreads();  // pl is free for the next item, red is complete
+    if (tid < 64) {
+      const double tot = wave0_sum(red);
+      if (tid == 0) kin_partial[st * geo.nxa + xi] = tot;
+    }
   }
 }
 
@@ -735,7 +748,8 @@ int launch_plane_fft_gather64(const double2* T, const double2* q,
                               int64_t state0, int nstates, const BoxGeom& geo,
                               const double2* tw64, const GatherParams& gp,
                               hipStream_t s) {
-  size_t shm = (size_t)(64 * kPL + 64) * sizeof(double2);
+  size_t shm = (size_t)(64 * kPL + 64) * sizeof(double2) +
+               kPlaneBlock * sizeof(double);
   const int64_t nitems = (int64_t)geo.nxa * nstates;
   const int64_t nblk = nitems < 1024 ? nitems : 1024;
   hipLaunchKernelGGL(plane_fft_gather64_kernel, dim3((unsigned)nblk),
