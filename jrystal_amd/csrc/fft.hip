@@ -175,6 +175,43 @@ constexpr int kXBlock = 256;
 constexpr int kXCap = 2048;
 constexpr int kXEpt = kXCap / kXBlock;
 
+// Streaming access to the plane buffer (written once and read once per
+// sweep: gigabytes, far beyond the caches) carries the nontemporal hint.
+__device__ __forceinline__ void nt_store(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ double2 nt_load(const double2* p) {
+  return make_double2(__builtin_nontemporal_load(&p->x),
+                      __builtin_nontemporal_load(&p->y));
+}
+
+// The x-line passes stage LZ lines of nx points in LDS, slot e = x * LZ + l
+// (line l = z - z0).  Thread t owns slots t + k * kXBlock; goff[k] is the
+// slot's offset in one state's plane buffer, or -1 when the slot is outside
+// the block's lines or its x-plane holds no mask point (those slots stay 0).
+__device__ __forceinline__ void xline_slots(int (&goff)[kXEpt],
+                                            const BoxGeom& geo, int nxl,
+                                            int LZ, int nl) {
+#pragma unroll
+  for (int k = 0; k < kXEpt; ++k) {
+    const int e = threadIdx.x + k * kXBlock;
+    const int x = e / LZ, l = e - x * LZ;
+    const int xi = (e < nxl && l < nl) ? geo.xmap[x] : -1;
+    goff[k] = xi >= 0 ? xi * geo.ps + l : -1;
+  }
+}
+
+// Registers of one state's slots (prefetched while the previous state's
+// lines are transformed).
+__device__ __forceinline__ void xline_load(double2 (&pre)[kXEpt],
+                                           const int (&goff)[kXEpt],
+                                           const double2* src) {
+#pragma unroll
+  for (int k = 0; k < kXEpt; ++k)
+    pre[k] = goff[k] >= 0 ? nt_load(src + goff[k]) : czero();
+}
+
 int xlines_per_block(int nx) {
   int LZ = kXCap / nx;
   if (LZ > kXLines) LZ = kXLines;
@@ -202,7 +239,6 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
   extern __shared__ double2 sm[];
   const int nx = NX > 0 ? NX : geo.nx;
   const int ny = geo.ny, nz = geo.nz, P = ny * nz;
-  int* px = reinterpret_cast<int*>(sm + nx * LZ);
   const int nzt = (nz + LZ - 1) / LZ;
   const int y = blockIdx.x / nzt;
   const int z0 = (blockIdx.x % nzt) * LZ;
@@ -211,19 +247,23 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
   const int per = (nstates + G - 1) / G;
   const int cb = gi * per, ce = min(nstates, cb + per);
   if (cb >= ce) return;  // uniform over the block
-  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
+  const int nxl = nx * LZ;
+  int goff[kXEpt];
+  xline_slots(goff, geo, nxl, LZ, nl);
+  const double2* src0 = T + (int64_t)y * nz + z0;
+  const int64_t sstride = (int64_t)geo.nxa * geo.ps;
+  double2 pre[kXEpt];
+  xline_load(pre, goff, src0 + (int64_t)cb * sstride);
   double acc[kXEpt];
 #pragma unroll
   for (int k = 0; k < kXEpt; ++k) acc[k] = 0.0;
-  const int nxl = nx * LZ;
   for (int c = cb; c < ce; ++c) {
-    for (int e = threadIdx.x; e < nxl; e += kXBlock) sm[e] = czero();
-    __syncthreads();
-    const double2* src = T + (int64_t)c * geo.nxa * geo.ps + (int64_t)y * nz + z0;
-    for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
-      const int l = e % nl, xi = e / nl;
-      sm[px[xi] * LZ + l] = src[(int64_t)xi * geo.ps + l];
+#pragma unroll
+    for (int k = 0; k < kXEpt; ++k) {
+      const int e = threadIdx.x + k * kXBlock;
+      if (e < nxl) sm[e] = pre[k];
     }
+    if (c + 1 < ce) xline_load(pre, goff, src0 + (int64_t)(c + 1) * sstride);
     __syncthreads();
     fft_lines<NX, true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
     const double f = occ[state0 + c];
@@ -286,8 +326,7 @@ __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
     BoxGeom geo, FftLen Lx, int LZ) {
   extern __shared__ double2 sm[];
   const int nx = NX > 0 ? NX : geo.nx;
-  const int ny = geo.ny, nz = geo.nz, P = ny * nz;
-  int* px = reinterpret_cast<int*>(sm + nx * LZ);
+  const int ny = geo.ny, nz = geo.nz;
   const int nzt = (nz + LZ - 1) / LZ;
   const int y = blockIdx.x / nzt;
   const int z0 = (blockIdx.x % nzt) * LZ;
@@ -296,8 +335,9 @@ __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
   const int per = (nstates + G - 1) / G;
   const int cb = gi * per, ce = min(nstates, cb + per);
   if (cb >= ce) return;  // uniform over the block
-  for (int i = threadIdx.x; i < geo.nxa; i += kXBlock) px[i] = geo.plane_x[i];
   const int nxl = nx * LZ;
+  int goff[kXEpt];
+  xline_slots(goff, geo, nxl, LZ, nl);
   double vr[kXEpt];
 #pragma unroll
   for (int k = 0; k < kXEpt; ++k) {
@@ -305,14 +345,17 @@ __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
     const int x = e / LZ, l = e % LZ;
     vr[k] = (e < nxl && l < nl) ? v[((int64_t)x * ny + y) * nz + z0 + l] : 0.0;
   }
+  double2* buf0 = T + (int64_t)y * nz + z0;
+  const int64_t sstride = (int64_t)geo.nxa * geo.ps;
+  double2 pre[kXEpt];
+  xline_load(pre, goff, buf0 + (int64_t)cb * sstride);
   for (int c = cb; c < ce; ++c) {
-    for (int e = threadIdx.x; e < nxl; e += kXBlock) sm[e] = czero();
-    __syncthreads();
-    double2* buf = T + (int64_t)c * geo.nxa * geo.ps + (int64_t)y * nz + z0;
-    for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
-      const int l = e % nl, xi = e / nl;
-      sm[px[xi] * LZ + l] = buf[(int64_t)xi * geo.ps + l];
+#pragma unroll
+    for (int k = 0; k < kXEpt; ++k) {
+      const int e = threadIdx.x + k * kXBlock;
+      if (e < nxl) sm[e] = pre[k];
     }
+    if (c + 1 < ce) xline_load(pre, goff, buf0 + (int64_t)(c + 1) * sstride);
     __syncthreads();
     fft_lines<NX, true, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
 #pragma unroll
@@ -322,10 +365,10 @@ __global__ __launch_bounds__(kXBlock) void xline_vpsi_kernel(
     }
     __syncthreads();
     fft_lines<NX, false, kXBlock, kXCap, true>(sm, nl, 1, LZ, nullptr, Lx);
-    for (int e = threadIdx.x; e < geo.nxa * nl; e += kXBlock) {
-      const int l = e % nl, xi = e / nl;
-      buf[(int64_t)xi * geo.ps + l] = sm[px[xi] * LZ + l];
-    }
+    double2* buf = buf0 + (int64_t)c * sstride;
+#pragma unroll
+    for (int k = 0; k < kXEpt; ++k)
+      if (goff[k] >= 0) nt_store(buf + goff[k], sm[threadIdx.x + k * kXBlock]);
     __syncthreads();
   }
 }
@@ -466,20 +509,12 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
 // no block barriers inside the transforms, loads and stores straight from
 // registers.  Plane rows are padded to 66 entries so that the column
 // accesses of 16-lane groups hit distinct LDS banks.
-// Streaming access to the plane buffer (written once and read once per
+// Streaming access (nt_load / nt_store) to the plane buffer (written once and read once per
 // sweep: gigabytes, far beyond the caches) and to the coefficients the
 // inverse plane pass scatters into LDS carries the nontemporal hint
 // (measured at 64^3: x-line passes 5-13% and the inverse plane pass 20%
 // faster).  Loads of data that another wave of the same kernel re-reads
 // (the QR Gram inputs) keep the default policy, which measured faster.
-__device__ __forceinline__ void nt_store(double2* p, double2 v) {
-  __builtin_nontemporal_store(v.x, &p->x);
-  __builtin_nontemporal_store(v.y, &p->y);
-}
-__device__ __forceinline__ double2 nt_load(const double2* p) {
-  return make_double2(__builtin_nontemporal_load(&p->x),
-                      __builtin_nontemporal_load(&p->y));
-}
 
 constexpr int kPL = 66;
 constexpr int kExLine = 72;  // padded exchange slots per x-line
