@@ -73,6 +73,10 @@ typedef struct {
   int xline_blocks;        /* target workgroups of an x-line pass, 0 = 4096 */
   int plane_pad;           /* extra elements per x-plane of the plane
                               buffer, -1 = automatic                        */
+  int qr_passes;           /* 0 = adaptive CholeskyQR: one pass for the
+                              k-points whose Gram matrix is close to
+                              diagonal (||D S D - I||_F < 0.5), two for the
+                              rest; 2 = always two passes                   */
 } jrx_problem;
 
 int jrx_plan_create(jrx_plan** plan, const jrx_problem* problem);

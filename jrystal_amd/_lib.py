@@ -28,7 +28,7 @@ class JrxProblem(ctypes.Structure):
     ("charges", ctypes.c_void_p), ("occupation", ctypes.c_void_p),
     ("xc", ctypes.c_int), ("chunk_states", ctypes.c_int),
     ("fft_variant", ctypes.c_int), ("xline_blocks", ctypes.c_int),
-    ("plane_pad", ctypes.c_int),
+    ("plane_pad", ctypes.c_int), ("qr_passes", ctypes.c_int),
   ]
 
 

@@ -87,6 +87,7 @@ struct jrx_plan {
   bool keep_planes = false;    // plane buffer holds all states
   int groups = 1;  // state groups of the x-line kernels
   int xline_blocks = 0;  // target workgroups of an x-line pass (0 = 4096)
+  bool qr_adaptive = true;  // one CholeskyQR pass where W is well conditioned
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
   // timing
@@ -96,7 +97,7 @@ struct jrx_plan {
   // workspace offsets (bytes)
   size_t o_stats;
   size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_tb, o_S, o_R1, o_R2, o_Ri, o_A,
-      o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, total;
+      o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, o_qrflag, total;
 };
 
 namespace {
@@ -296,6 +297,9 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   JRX_REQUIRE(d->fft_variant == 0 || d->fft_variant == 1,
               "fft_variant must be 0 (automatic) or 1 (runtime radix)");
   JRX_REQUIRE(d->xline_blocks >= 0, "xline_blocks must be >= 0");
+  JRX_REQUIRE(d->qr_passes == 0 || d->qr_passes == 2,
+              "qr_passes must be 0 (adaptive) or 2");
+  p->qr_adaptive = d->qr_passes == 0;
   p->force_generic = (gen && gen[0] == '1') || d->fft_variant == 1;
   p->xline_blocks = d->xline_blocks > 0 ? d->xline_blocks : 0;
   p->groups = xline_groups(p->geo,
@@ -335,6 +339,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_ee = take((size_t)p->rec_parts * sizeof(double));
   p->o_exc = take((size_t)p->rec_parts * sizeof(double));
   p->o_stats = take((size_t)(2 * kStatBlocks + 8) * sizeof(double));
+  p->o_qrflag = take((size_t)p->P * sizeof(int));
   p->total = off;
   return JRX_OK;
 }
@@ -954,13 +959,18 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
     if ((rc = (x))) return rc;                          \
     if ((rc = timing_end(p, s, slot, cls))) return rc;  \
   } while (0)
-  // --- QR (CholeskyQR2) ----------------------------------------------------
+  // --- QR (CholeskyQR2; adaptive: the second pass only for the k-points
+  // whose first Gram matrix is far from diagonal) --------------------------
+  int* one_pass = p->qr_adaptive
+                      ? reinterpret_cast<int*>(
+                            static_cast<char*>(workspace) + p->o_qrflag)
+                      : nullptr;
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, p->gsw, s));
   JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
-  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s));
-  JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s));
+  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s, one_pass));
+  JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s, one_pass));
   JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
-  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s));
+  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s, nullptr, one_pass));
   JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));
   JRX_T(JRX_K_QR_APPLY, launch_apply_fwd(w_re, w_im, Ri, q, P, ng, nb, s));
   // --- density -------------------------------------------------------------

@@ -56,7 +56,7 @@ template <int T, bool PRE>
 __global__ __launch_bounds__(kQrBlock) void gram_w_kernel(
     const double* __restrict__ wr, const double* __restrict__ wi,
     const double2* __restrict__ pre, double2* __restrict__ partial, int ng,
-    int nb, int rows_per_chunk, int nchunk) {
+    int nb, int rows_per_chunk, int nchunk, const int* __restrict__ skip) {
   extern __shared__ double2 sm[];
   constexpr int NBP = 16 * T;
   constexpr int NP = T * (T + 1) / 2;
@@ -64,6 +64,7 @@ __global__ __launch_bounds__(kQrBlock) void gram_w_kernel(
   const int li = lane & 15, lk = lane >> 4;
   const int64_t p = blockIdx.y;
   const int chunk = blockIdx.x;
+  if (skip && skip[p]) return;  // uniform over the block
   if (PRE) load_small(sm, pre + p * nb * nb, nb, NBP);
   __syncthreads();
   d4 sre[NP], sim[NP];
@@ -160,18 +161,19 @@ __global__ __launch_bounds__(kQrBlock) void gram_w_kernel(
 
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
-                       const GramSplit& sp, hipStream_t s) {
+                       const GramSplit& sp, hipStream_t s, const int* skip) {
   const int T = (nb + 15) / 16;
   size_t shm = (pre ? 2 : 1) * (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
 #define JRX_GRAM_W(TT)                                                        \
   if (pre)                                                                    \
     hipLaunchKernelGGL((gram_w_kernel<TT, true>), grid, block, shm, s, wr, wi, \
-                       pre, partial, ng, nb, sp.rows_per_chunk, sp.nchunk);   \
+                       pre, partial, ng, nb, sp.rows_per_chunk, sp.nchunk,    \
+                       skip);                                                 \
   else                                                                        \
     hipLaunchKernelGGL((gram_w_kernel<TT, false>), grid, block, shm, s, wr,   \
                        wi, pre, partial, ng, nb, sp.rows_per_chunk,           \
-                       sp.nchunk);
+                       sp.nchunk, skip);
   switch (T) {
     case 1: JRX_GRAM_W(1) break;
     case 2: JRX_GRAM_W(2) break;
@@ -297,19 +299,42 @@ int launch_reduce_partials(const double2* partial, double2* out, int64_t P,
 
 // One block per problem; A and X live in LDS ([nb][nb+1] each).
 __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
-    const double2* __restrict__ S, double2* __restrict__ rinv, int nb) {
+    const double2* __restrict__ S, double2* __restrict__ rinv, int nb,
+    int* __restrict__ flag_out, const int* __restrict__ flag_in) {
   extern __shared__ double2 sm[];
   const int ld = nb + 1;
   double2* A = sm;
   double2* X = sm + nb * ld;
   const int64_t p = blockIdx.x;
   const int tid = threadIdx.x;
+  double2* out = rinv + p * nb * nb;
+  if (flag_in && flag_in[p]) {  // uniform over the block
+    for (int e = tid; e < nb * nb; e += kQrBlock) {
+      const int i = e / nb, j = e - i * nb;
+      out[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    return;
+  }
   const double2* Sp = S + p * nb * nb;
   for (int e = tid; e < nb * nb; e += kQrBlock) {
     const int i = e / nb, j = e - i * nb;
     A[i * ld + j] = Sp[e];
   }
   __syncthreads();
+  if (flag_out) {  // distance of the scaled Gram matrix from I
+    double a = 0.0;
+    for (int e = tid; e < nb * nb; e += kQrBlock) {
+      const int i = e / nb, j = e - i * nb;
+      if (i != j) {
+        const double2 v = A[i * ld + j];
+        a += (v.x * v.x + v.y * v.y) / (A[i * ld + i].x * A[j * ld + j].x);
+      }
+    }
+    // X is free until the back substitution
+    const double d2 = block_sum(a, reinterpret_cast<double*>(X));
+    if (tid == 0)  // NaN or a non-positive diagonal compares false
+      flag_out[p] = (d2 < kQrOnePass * kQrOnePass) ? 1 : 0;
+  }
   for (int j = 0; j < nb; ++j) {
     if (tid == 0) {
       const double d = A[j * ld + j].x;
@@ -338,7 +363,6 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
     }
   }
   __syncthreads();
-  double2* out = rinv + p * nb * nb;
   for (int e = tid; e < nb * nb; e += kQrBlock) {
     const int i = e / nb, j = e - i * nb;
     out[e] = (j >= i) ? X[i * ld + j] : czero();
@@ -346,10 +370,10 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
 }
 
 int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
-                    hipStream_t s) {
+                    hipStream_t s, int* flag_out, const int* flag_in) {
   size_t shm = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
   hipLaunchKernelGGL(chol_inv_kernel, dim3((unsigned)P), dim3(kQrBlock), shm,
-                     s, S, rinv, nb);
+                     s, S, rinv, nb, flag_out, flag_in);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

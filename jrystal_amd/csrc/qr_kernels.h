@@ -23,9 +23,12 @@ GramSplit gram_split(int64_t P, int ng);
 // partial[p][c] = sum over the rows of chunk c of conj(a_g)^T a_g,
 // a_g = W[p][g][:] (times pre[p] on the right when pre != nullptr): one
 // partial per chunk (the block's four waves are summed in LDS).
+// With skip, problems p with skip[p] != 0 are left out (their partials are
+// not written).
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
-                       const GramSplit& sp, hipStream_t s);
+                       const GramSplit& sp, hipStream_t s,
+                       const int* skip = nullptr);
 
 // partial[p][c] = sum over the rows of chunk c of
 // conj(Q[p][:][g]) (x) Qb[p][:][g] (one partial per chunk).
@@ -39,8 +42,16 @@ int launch_reduce_partials(const double2* partial, double2* out, int64_t P,
 
 // S = R^H R (R upper, real positive diagonal); writes R^{-1}.  A
 // non-positive pivot turns the result into NaN.
+// flag_out (optional): flag_out[p] = 1 when S is close enough to diagonal
+// for one CholeskyQR pass to be accurate, namely
+//   sum_{i != j} |S_ij|^2 / (S_ii S_jj) < kQrOnePass^2,
+// i.e. ||D S D - I||_F < kQrOnePass with D = diag(S)^-1/2, which bounds
+// cond(D S D) by (1 + x) / (1 - x) = 3 and cond(W D) by sqrt(3); else 0.
+// flag_in (optional): problems with flag_in[p] != 0 get R^{-1} = I.
+constexpr double kQrOnePass = 0.5;
 int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
-                    hipStream_t s);
+                    hipStream_t s, int* flag_out = nullptr,
+                    const int* flag_in = nullptr);
 
 // c = a * b for upper-triangular a, b.
 int launch_trmm_uu(const double2* a, const double2* b, double2* c, int64_t P,

@@ -42,13 +42,15 @@ class EnergyPlan:
     fft_variant: 0 automatic, 1 runtime-radix FFT kernels everywhere.
     xline_blocks: target workgroups of an x-line pass (0 = 4096).
     plane_pad: extra elements per x-plane of the plane buffer (-1 =
-    automatic).
+    automatic).  qr_passes: 0 = adaptive CholeskyQR (one pass for the
+    k-points whose Gram matrix is close to diagonal, two for the rest), 2 =
+    always two passes.
   """
 
   def __init__(self, positions, charges, cell_vectors, mask, kpts,
                occupation, xc: str = "lda_x", chunk_states: int = 0,
                device=None, fft_variant: int = 0, xline_blocks: int = 0,
-               plane_pad: int = -1):
+               plane_pad: int = -1, qr_passes: int = 0):
     _require_cuda()
     if xc != "lda_x":
       raise NotImplementedError(f"xc {xc!r}: only lda_x is on this path")
@@ -85,10 +87,12 @@ class EnergyPlan:
     d.fft_variant = int(fft_variant)
     d.xline_blocks = int(xline_blocks)
     d.plane_pad = int(plane_pad)
+    d.qr_passes = int(qr_passes)
     self.options = {"chunk_states": int(chunk_states),
                     "fft_variant": int(fft_variant),
                     "xline_blocks": int(xline_blocks),
-                    "plane_pad": int(plane_pad)}
+                    "plane_pad": int(plane_pad),
+                    "qr_passes": int(qr_passes)}
     h = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(self._lib.jrx_plan_create(ctypes.byref(h), ctypes.byref(d)))

@@ -201,3 +201,46 @@ def test_compile_time_radix_sizes(n):
   wl = W.build(f"si2-{n}^3", W.fcc_si2(4.0), (n, n, n), (1, 1, 2), 8,
                W.alias_free_cutoff(W.fcc_si2(4.0).cell_vectors, (n, n, n)))
   _case(wl, seed=n)
+
+
+@pytest.mark.parametrize("cond", ["random", "ill"])
+def test_adaptive_cholesky_qr(cond):
+  """Adaptive CholeskyQR (one pass for the k-points whose scaled Gram
+  matrix is within 0.5 of I in Frobenius norm, two for the rest) against
+  two passes everywhere and against the oracle (Householder QR).  A random
+  start takes one pass; a start whose bands 1..nb-1 lean towards band 0
+  (cond(W) ~ 1e4: one pass would leave ~eps cond^2 of non-orthogonality)
+  must take two, i.e. give the two-pass result bit for bit."""
+  from jrystal_amd import workloads as W
+  from jrystal_amd.objective import EnergyPlan
+  wl = W.si2_fcc(4.0)
+  prob = oracle_problem(wl.crystal, wl.grid_sizes, wl.k_grid, wl.num_bands,
+                        wl.cutoff_energy)
+  params = seeded_params(wl.shape, 7)
+  if cond == "ill":
+    for k in ("w_re", "w_im"):
+      w = params[k]
+      w[..., 1:] = w[..., :1] + 1e-4 * w[..., 1:]
+  outs = {}
+  for qp in (0, 2):
+    cr = wl.crystal
+    plan = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, wl.mask,
+                      wl.kpts, wl.occupation, qr_passes=qp)
+    outs[qp] = _run(plan, params)
+    plan.close()
+  _check(outs[0], prob, params)
+  for k in ("energies", "dw_re", "dw_im", "density"):
+    if cond == "ill":
+      assert np.array_equal(outs[0][k], outs[2][k]), k
+    else:
+      assert rel_max(outs[0][k], outs[2][k]) < 1e-12, k
+
+
+def test_qr_passes_option_is_checked():
+  from jrystal_amd import workloads as W
+  from jrystal_amd.objective import EnergyPlan
+  wl = W.tiny()
+  cr = wl.crystal
+  with pytest.raises(ValueError):
+    EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, wl.mask, wl.kpts,
+               wl.occupation, qr_passes=1)
