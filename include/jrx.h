@@ -125,6 +125,23 @@ int jrx_plan_precondition(jrx_plan* plan, const double* g_re,
 int jrx_plan_band_norms(const jrx_plan* plan, double* out,
                         const void* workspace, size_t workspace_bytes,
                         void* stream);
+
+/* Band energies: with the switch on, every energy+gradient evaluation also
+ * forms dE/df[s,k,b] = <psi|H|psi> = 1/2 sum_G |G+k|^2 |Q_Gb|^2 +
+ * (Omega/N) sum_r v(r) |psi_b(r)|^2, v = v_H + v_ext + v_xc, for every band
+ * (occupied or not); jrx_plan_band_energies writes the last one to out[S]
+ * (device).  The gradient of the energy with respect to the occupations of
+ * the reference's parameterised occupations (occupation.py:261-278, used by
+ * calc_ground_state_energy_all_electrons.py:139-181) needs it. */
+int jrx_plan_set_band_energies(jrx_plan* plan, int on);
+int jrx_plan_band_energies(const jrx_plan* plan, double* out,
+                           const void* workspace, size_t workspace_bytes,
+                           void* stream);
+/* Replaces the plan's occupation [S] (device or host memory, stream-ordered
+ * copy): the occupations of the reference's parameterised occupation methods
+ * change every optimiser step. */
+int jrx_plan_set_occupation(jrx_plan* plan, const double* occupation,
+                            void* stream);
 int jrx_plan_orthonormal_params(const jrx_plan* plan, double* out_re,
                                 double* out_im, const void* workspace,
                                 size_t workspace_bytes, void* stream);

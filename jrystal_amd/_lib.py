@@ -104,6 +104,9 @@ SIGNATURES = {
   "jrx_ewald": [_P, _P, _I, _P, _I64, _D, _D, _P, _I64, _P, _P, _SZ, _P],
   "jrx_plan_precondition": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_band_norms": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_set_band_energies": [_P, _I],
+  "jrx_plan_band_energies": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_set_occupation": [_P, _P, _P],
   "jrx_plan_orthonormal_params": [_P, _P, _P, _P, _SZ, _P],
   "jrx_plan_subspace_matrix": [_P, _P, _P, _SZ, _P],
   "jrx_plan_rotate": [_P, _P, _P, _P, _P, _SZ, _P],

@@ -72,18 +72,18 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   if config.use_pseudopotential:
     raise NotImplementedError("pseudopotential energies are outside the "
                               "all-electron path")
-  if config.occupation not in ("uniform", "gamma"):
-    raise NotImplementedError(
-      f"occupation {config.occupation!r}: only the fixed uniform / gamma "
-      "occupations are on this path")
+  if config.occupation not in ("uniform", "gamma") + occupation.PARAMETERISED:
+    raise ValueError(f"Invalid method: {config.occupation}")
+  parameterised = config.occupation in occupation.PARAMETERISED
   if config.optimizer not in ("adam", "cg", "cg-learned"):
     raise NotImplementedError(
       f"optimizer {config.optimizer!r}: adam (the reference's), cg "
       "(preconditioned conjugate gradients) or cg-learned (with the learned "
       "preconditioner) are on this path")
-  if config.optimizer != "adam" and config.smearing > 0.0:
+  if config.optimizer != "adam" and (config.smearing > 0.0 or parameterised):
     raise NotImplementedError("the cg optimisers minimise the energy at "
-                              "fixed occupations: set smearing to 0")
+                              "fixed occupations: set smearing to 0 and "
+                              "occupation to uniform or gamma")
   if config.scheduler:
     raise NotImplementedError("Scheduler is not implemented yet.")
   cr = create_crystal(config)
@@ -98,8 +98,16 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   ew = float(energy.nuclear_repulsion(
     cr.positions, cr.charges, cr.cell_vectors, g_vec, cr.vol,
     config.ewald_args["ewald_eta"], config.ewald_args["ewald_cutoff"]))
-  occ = occupation.param_init(None, num_bands, ne, nk, cr.spin,
-                              config.occupation, config.spin_restricted)
+  occ_params = occupation.param_init(config.seed, num_bands, ne, nk, cr.spin,
+                                     config.occupation, config.spin_restricted)
+
+  def get_occupation(p):
+    return occupation.occupation(p, nk, ne, cr.spin, config.occupation,
+                                 config.spin_restricted)
+  if parameterised:
+    occ = get_occupation(occ_params).detach().cpu().numpy()
+  else:
+    occ = occ_params
   etro = entropy.fermi_dirac(occ, eps=config.eps)
   plan = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, mask, kpts,
                     occ, xc=config.xc, chunk_states=config.chunk_states)
@@ -109,7 +117,15 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   if config.optimizer != "adam":
     return _calc_cg(config, cr, plan, params, occ, ew)
   opt_args = dict(config.optimizer_args)
-  opt = Adam(params, **opt_args)
+  if parameterised:
+    # one optimiser over both parameter sets, as the reference's update
+    # (calc_ground_state_energy_all_electrons.py:167-181)
+    occ_flat, occ_rebuild = occupation.leaves(occ_params)
+    plan.set_band_energies(True)
+    leaves = dict(params, **occ_flat)
+  else:
+    leaves = params
+  opt = Adam(leaves, **opt_args)
   if config.smearing > 0.0:
     temperature = exponential_decay(100.0, config.epoch // 2,
                                     config.smearing / 100, config.smearing)
@@ -127,8 +143,23 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   i = 0
   for i in range(config.epoch):
     temp = temperature(i)
-    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
-    opt.step(params, {"w_re": out["dw_re"], "w_im": out["dw_im"]})
+    if parameterised:
+      # F = E(W, f) - T S(f): dF/dW from the plan; dF/df = band energies
+      # - T dS/df, pulled back through the occupation map by autograd
+      view = {k: v.detach().requires_grad_() for k, v in occ_flat.items()}
+      occ_t = get_occupation(occ_rebuild(view))
+      plan.set_occupation(occ_t.detach())
+      plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+      s_t = entropy.fermi_dirac(occ_t, eps=config.eps)
+      surrogate = (plan.band_energies() * occ_t).sum() - temp * s_t
+      g_occ = torch.autograd.grad(surrogate, list(view.values()))
+      grads = {"w_re": out["dw_re"], "w_im": out["dw_im"]}
+      grads.update({k: g.contiguous() for k, g in zip(view, g_occ)})
+      opt.step(leaves, grads)
+      etro = float(s_t.detach())
+    else:
+      plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+      opt.step(params, {"w_re": out["dw_re"], "w_im": out["dw_im"]})
     etot = float(out["energies"][4])  # one host sync per step
     history.append(etot)
     if not np.isfinite(etot):
@@ -143,6 +174,11 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   seconds = time.perf_counter() - t0
   if not converged:
     log.warning("Did not converge.")
+  if parameterised:
+    with torch.no_grad():
+      plan.set_occupation(get_occupation(occ_rebuild(occ_flat)))
+    return _final(config, cr, plan, params, occ_rebuild(occ_flat), ew,
+                  history, converged, i + 1, seconds)
   return _final(config, cr, plan, params, occ, ew, history, converged, i + 1,
                 seconds)
 

@@ -450,7 +450,8 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
                az = mx * gp.B[2] + kz;
   const double2* qs = q + st * geo.ng;
   double2* qb = qbar + st * geo.ng;
-  double kin = 0.0;
+  const bool be = gp.pot_partial != nullptr;
+  double kin = 0.0, pe = 0.0;
   for (int g = g0 + threadIdx.x; g < g1; g += kPlaneBlock) {
     const int yz = geo.g_yz[g];
     const int iy = yz / nz, iz = yz - iy * nz;
@@ -464,14 +465,20 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_fft_gather_kernel(
     qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
                          f * (gp.two_over_vol * hv.y + gk2 * cv.y));
     kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
+    pe += cv.x * hv.x + cv.y * hv.y;
   }
   __syncthreads();  // the plane is dead: reuse its LDS for the reduction
-  double* red = reinterpret_cast<double*>(sm);
+  double* red = reinterpret_cast<double*>(sm);  // [2][kPlaneBlock]
   red[threadIdx.x] = kin;
+  if (be) red[kPlaneBlock + threadIdx.x] = pe;
   __syncthreads();
   if (threadIdx.x < 64) {
     const double tot = wave0_sum(red);
-    if (threadIdx.x == 0) kin_partial[st * geo.nxa + xi] = tot;
+    const double tpe = be ? wave0_sum(red + kPlaneBlock) : 0.0;
+    if (threadIdx.x == 0) {
+      kin_partial[st * geo.nxa + xi] = tot;
+      if (be) gp.pot_partial[st * geo.nxa + xi] = tpe;
+    }
   }
 }
 
@@ -482,7 +489,8 @@ int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
                             hipStream_t s) {
   const int P = geo.ny * geo.nz;
   size_t shm = (size_t)P * sizeof(double2);
-  if (shm < kPlaneBlock * sizeof(double)) shm = kPlaneBlock * sizeof(double);
+  if (shm < 2 * kPlaneBlock * sizeof(double))  // the final reductions
+    shm = 2 * kPlaneBlock * sizeof(double);
   dim3 grid(geo.nxa, nstates), block(kPlaneBlock);
   const int cube = geo.ny == geo.nz ? geo.ny : 0;
 #define JRX_PG(NN)                                                          \
@@ -568,6 +576,7 @@ __global__ __launch_bounds__(kPlaneBlock) void plane_ifft64_kernel(
 // measured 7% faster than one block per item).
 // Two blocks fit a CU's LDS (72.7 KB each); both are resident only at 4
 // waves per SIMD, i.e. at most 128 VGPRs, hence the second bound.
+template <bool BE>
 __global__ __launch_bounds__(kPlaneBlock, 4) void plane_fft_gather64_kernel(
     const double2* __restrict__ T, const double2* __restrict__ q,
     double2* __restrict__ qbar, double* __restrict__ kin_partial,
@@ -576,7 +585,8 @@ __global__ __launch_bounds__(kPlaneBlock, 4) void plane_fft_gather64_kernel(
   extern __shared__ double2 sm[];
   double2* pl = sm;
   double2* stw = sm + 64 * kPL;
-  double* red = reinterpret_cast<double*>(sm + 64 * kPL + 64);  // [512]
+  // [512] kinetic, then (BE) [512] band-energy partials
+  double* red = reinterpret_cast<double*>(sm + 64 * kPL + 64);
   const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
   const int nitems = geo.nxa * nstates;
   if (tid < 64) stw[tid] = tw64[tid];
@@ -629,7 +639,7 @@ __global__ __launch_bounds__(kPlaneBlock, 4) void plane_fft_gather64_kernel(
     const double az = mx * gp.B[2] + gp.kpts[3 * k + 2];
     const double2* qs = q + st * geo.ng;
     double2* qb = qbar + st * geo.ng;
-    double kin = 0.0;
+    double kin = 0.0, pe = 0.0;
     for (int g = g0 + tid; g < g1; g += kPlaneBlock) {
       const int yz = geo.g_yz[g];
       const int iy = yz >> 6, iz = yz & 63;
@@ -643,14 +653,20 @@ __global__ __launch_bounds__(kPlaneBlock, 4) void plane_fft_gather64_kernel(
       qb[g] = make_double2(f * (gp.two_over_vol * hv.x + gk2 * cv.x),
                            f * (gp.two_over_vol * hv.y + gk2 * cv.y));
       kin += gk2 * (cv.x * cv.x + cv.y * cv.y);
+      if constexpr (BE) pe += cv.x * hv.x + cv.y * hv.y;
     }
     // red lies outside pl; wave 0 reads it before its next barrier, and no
     // thread stores to it again before passing that barrier
     red[tid] = kin;
+    if constexpr (BE) red[kPlaneBlock + tid] = pe;
     __syncthreads();  // pl is free for the next item, red is complete
     if (tid < 64) {
       const double tot = wave0_sum(red);
       if (tid == 0) kin_partial[st * geo.nxa + xi] = tot;
+      if constexpr (BE) {
+        const double tpe = wave0_sum(red + kPlaneBlock);
+        if (tid == 0) gp.pot_partial[st * geo.nxa + xi] = tpe;
+      }
     }
   }
 }
@@ -783,13 +799,19 @@ int launch_plane_fft_gather64(const double2* T, const double2* q,
                               int64_t state0, int nstates, const BoxGeom& geo,
                               const double2* tw64, const GatherParams& gp,
                               hipStream_t s) {
+  const bool be = gp.pot_partial != nullptr;
   size_t shm = (size_t)(64 * kPL + 64) * sizeof(double2) +
-               kPlaneBlock * sizeof(double);
+               (be ? 2 : 1) * kPlaneBlock * sizeof(double);
   const int64_t nitems = (int64_t)geo.nxa * nstates;
   const int64_t nblk = nitems < 1024 ? nitems : 1024;
-  hipLaunchKernelGGL(plane_fft_gather64_kernel, dim3((unsigned)nblk),
-                     dim3(kPlaneBlock), shm, s, T, q, qbar, kin_partial,
-                     state0, nstates, geo, tw64, gp);
+  if (be)
+    hipLaunchKernelGGL(plane_fft_gather64_kernel<true>, dim3((unsigned)nblk),
+                       dim3(kPlaneBlock), shm, s, T, q, qbar, kin_partial,
+                       state0, nstates, geo, tw64, gp);
+  else
+    hipLaunchKernelGGL(plane_fft_gather64_kernel<false>, dim3((unsigned)nblk),
+                       dim3(kPlaneBlock), shm, s, T, q, qbar, kin_partial,
+                       state0, nstates, geo, tw64, gp);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

@@ -69,6 +69,9 @@ struct GatherParams {
   double B[9];         // reciprocal basis rows
   int nk, nb;          // state -> k index = (state / nb) % nk
   double two_over_vol;
+  // optional: pot_partial[state * nxa + plane] = Re sum_g conj(q_g) h_g over
+  // the plane's mask points, h = FFT(v psi) (band energies)
+  double* pot_partial;
 };
 int launch_plane_fft_gather(const double2* T, const double2* q, double2* qbar,
                             double* kin_partial, int64_t state0, int nstates,

@@ -88,6 +88,7 @@ struct jrx_plan {
   int groups = 1;  // state groups of the x-line kernels
   int xline_blocks = 0;  // target workgroups of an x-line pass (0 = 4096)
   bool qr_adaptive = true;  // one CholeskyQR pass where W is well conditioned
+  bool band_e = false;      // energy+gradient evaluations also form dE/df
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
   // timing
@@ -97,7 +98,8 @@ struct jrx_plan {
   // workspace offsets (bytes)
   size_t o_stats;
   size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_tb, o_S, o_R1, o_R2, o_Ri, o_A,
-      o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, o_qrflag, total;
+      o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, o_qrflag, o_bepart, o_pot,
+      total;
 };
 
 namespace {
@@ -340,6 +342,8 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_exc = take((size_t)p->rec_parts * sizeof(double));
   p->o_stats = take((size_t)(2 * kStatBlocks + 8) * sizeof(double));
   p->o_qrflag = take((size_t)p->P * sizeof(int));
+  p->o_bepart = take((size_t)p->S * p->geo.nxa * sizeof(double));
+  p->o_pot = take((size_t)p->S * sizeof(double));
   p->total = off;
   return JRX_OK;
 }
@@ -592,6 +596,45 @@ extern "C" int jrx_plan_band_norms(const jrx_plan* p, double* out,
       out, static_cast<const char*>(workspace) + p->o_tb,
       (size_t)p->S * sizeof(double), hipMemcpyDeviceToDevice,
       (hipStream_t)stream));
+  return JRX_OK;
+}
+
+__global__ void band_energy_kernel(const double* __restrict__ tb,
+                                   const double* __restrict__ pot,
+                                   double two_over_vol, int64_t S,
+                                   double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S) out[i] = 0.5 * (tb[i] + two_over_vol * pot[i]);
+}
+
+extern "C" int jrx_plan_set_band_energies(jrx_plan* p, int on) {
+  JRX_REQUIRE(p, "null plan");
+  p->band_e = on != 0;
+  return JRX_OK;
+}
+
+extern "C" int jrx_plan_band_energies(const jrx_plan* p, double* out,
+                                      const void* workspace,
+                                      size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && out && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  JRX_REQUIRE(p->band_e, "band energies are off (jrx_plan_set_band_energies)");
+  const char* ws = static_cast<const char*>(workspace);
+  hipLaunchKernelGGL(band_energy_kernel, dim3(ceil_div(p->S, 256)), dim3(256),
+                     0, (hipStream_t)stream,
+                     reinterpret_cast<const double*>(ws + p->o_tb),
+                     reinterpret_cast<const double*>(ws + p->o_pot),
+                     2.0 / p->vol, p->S, out);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+extern "C" int jrx_plan_set_occupation(jrx_plan* p, const double* occupation,
+                                       void* stream) {
+  JRX_REQUIRE(p && occupation, "null argument");
+  JRX_CHECK_HIP(hipMemcpyAsync(p->d_occ, occupation,
+                               (size_t)p->S * sizeof(double),
+                               hipMemcpyDefault, (hipStream_t)stream));
   return JRX_OK;
 }
 
@@ -1022,6 +1065,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
     gp.nk = p->nk;
     gp.nb = nb;
     gp.two_over_vol = 2.0 / p->vol;
+    gp.pot_partial = p->band_e ? dptr(p->o_bepart) : nullptr;
     const int64_t step = p->keep_planes ? SS : p->chunk;
     for (int sp = 0; sp < p->ns; ++sp) {
       for (int64_t s0 = 0; s0 < SS; s0 += step) {
@@ -1053,6 +1097,9 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
       }
     }
     JRX_T(JRX_K_OTHER, launch_state_sums(kin, p->geo.nxa, S, tb, s));
+    if (p->band_e)
+      JRX_T(JRX_K_OTHER, launch_state_sums(dptr(p->o_bepart), p->geo.nxa, S,
+                                           dptr(p->o_pot), s));
     // --- QR adjoint ----------------------------------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
     JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsq.nchunk, nb, s));

@@ -205,6 +205,37 @@ class EnergyPlan:
         _lib.stream_ptr(self.device)))
     return out
 
+  def set_band_energies(self, on: bool = True):
+    """Switch on (or off) the band energies of energy+gradient
+    evaluations (band_energies())."""
+    _lib.check(self._lib.jrx_plan_set_band_energies(self._plan, int(bool(on))))
+
+  def band_energies(self, out=None):
+    """dE/df[s,k,b] = <psi_b|H|psi_b> (kinetic + v_H + v_ext + v_xc) of the
+    last energy+gradient evaluation, for every band, device tensor
+    (ns,nk,nb); needs set_band_energies(True) before that evaluation."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb), dtype=torch.float64,
+                        device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_band_energies(
+        self._plan, _lib.ptr(out), self._ws_ptr, self.workspace_bytes,
+        _lib.stream_ptr(self.device)))
+    return out
+
+  def set_occupation(self, occupation):
+    """Replace the occupation (ns,nk,nb) used by later evaluations."""
+    occ = torch.as_tensor(occupation, dtype=torch.float64)
+    if tuple(occ.shape) != (self.ns, self.nk, self.nb):
+      raise ValueError(f"occupation must have shape {(self.ns, self.nk, self.nb)}"
+                       f", got {tuple(occ.shape)}")
+    occ = occ.to(self.device).contiguous()
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_set_occupation(
+        self._plan, _lib.ptr(occ), _lib.stream_ptr(self.device)))
+    self._occ_dev = occ  # keep the source alive until the copy has run
+    self.occupation = occ.detach().cpu().numpy()
+
   def orthonormal_params(self, out_re=None, out_im=None):
     """Q of the last evaluation in the (s,k,g,b) parameter layout (real and
     imaginary parts).  E(Q) = E(W), and Q has R = I."""

@@ -15,7 +15,9 @@ Modules (each function cites the reference file:line it follows):
 * ``oracle.pw``         -- QR orthonormalisation, scatter/gather, wave/density
 * ``oracle.energy``     -- kinetic / Hartree / external / LDA-x / Ewald / total
 * ``oracle.adjoint``    -- closed-form gradient dE/d(w_re, w_im)
-* ``oracle.occupation`` -- fixed occupations (uniform, gamma)
+* ``oracle.occupation`` -- fixed (uniform, gamma) and parameterised
+                            (simplex projector, idempotent) occupations,
+                            Fermi-Dirac entropy and their gradients
 * ``oracle.workloads``  -- the seeded synthetic inputs used by tests and bench
 * ``oracle.optim``      -- optimiser restatements (optax-style Adam,
                             convergence window, Teter-preconditioned descent)

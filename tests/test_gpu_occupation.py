@@ -1,0 +1,126 @@
+"""Occupation-dependent path on the GPU: occupations replaced on a plan
+(energies against the oracle), band energies dE/df against central
+differences of the oracle energy, the free-energy gradient with respect to
+the simplex-projector parameters against central differences of the GPU
+free energy, and the driver with parameterised occupations."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from helpers import oracle_problem, seeded_params
+from oracle import adjoint as OA
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _setup(seed=3):
+  from jrystal_amd import workloads as W
+  from jrystal_amd.objective import EnergyPlan
+  wl = W.si2_fcc(4.0)
+  prob = oracle_problem(wl.crystal, wl.grid_sizes, wl.k_grid, wl.num_bands,
+                        wl.cutoff_energy)
+  params = seeded_params(wl.shape, seed)
+  cr = wl.crystal
+  plan = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, wl.mask,
+                    wl.kpts, wl.occupation)
+  return wl, prob, params, plan
+
+
+def _energy(plan, params):
+  r = plan.evaluate(params["w_re"], params["w_im"], grad=False)
+  return float(r["energies"][4])
+
+
+def test_set_occupation_and_band_energies_match_oracle():
+  wl, prob, params, plan = _setup()
+  ns, nk, nb = wl.occupation.shape
+  rng = np.random.default_rng(4)
+  f = rng.uniform(0.0, 2.0 / nk, (ns, nk, nb))
+  plan.set_occupation(f)
+  plan.set_band_energies(True)
+  r = plan.evaluate(params["w_re"], params["w_im"], grad=True)
+  eb = plan.band_energies().cpu().numpy()
+  pf = dataclasses.replace(prob, occupation=f)
+  parts, _, _ = OA.energy_and_grad(params, pf, want_grad=False)
+  assert abs(float(r["energies"][4]) - parts["total"]) < 1e-9
+  h = 1e-5
+  for idx in [(0, 0, 0), (0, nk - 1, nb - 1), (0, 1, nb // 2)]:
+    e = np.zeros_like(f)
+    e[idx] = h
+    ep = OA.energy_and_grad(params, dataclasses.replace(prob, occupation=f + e),
+                            want_grad=False)[0]["total"]
+    em = OA.energy_and_grad(params, dataclasses.replace(prob, occupation=f - e),
+                            want_grad=False)[0]["total"]
+    fd = (ep - em) / (2 * h)
+    print(idx, eb[idx], fd)
+    assert abs(eb[idx] - fd) < 1e-6 * max(1.0, abs(fd))
+  # band energies do not depend on the occupation of empty bands: a band
+  # with f = 0 still gets its <psi|H|psi>
+  f0 = f.copy()
+  f0[0, 0, nb - 1] = 0.0
+  plan.set_occupation(f0)
+  plan.evaluate(params["w_re"], params["w_im"], grad=True)
+  assert np.isfinite(plan.band_energies().cpu().numpy()[0, 0, nb - 1])
+  with pytest.raises(ValueError):
+    plan.set_occupation(f[:, :, :-1])
+
+
+@pytest.mark.parametrize("method", ["simplex-projector", "idempotent"])
+def test_free_energy_gradient_wrt_occupation_params(method):
+  from jrystal_amd import entropy, occupation
+  wl, prob, params, plan = _setup(5)
+  ns, nk, nb = wl.occupation.shape
+  ne = 8
+  temp = 0.05
+  occ_params = occupation.param_init(1, nb, ne, nk, 0, method)
+  flat, rebuild = occupation.leaves(occ_params)
+  rng = np.random.default_rng(8)
+  if method == "simplex-projector":  # move off the symmetric start
+    for v in flat.values():
+      v += torch.tensor(rng.standard_normal(tuple(v.shape)), device=v.device)
+
+  def occ_of(view):
+    return occupation.occupation(rebuild(view), nk, ne, 0, method)
+
+  def free(view):
+    with torch.no_grad():
+      f = occ_of(view)
+      plan.set_occupation(f)
+      return _energy(plan, params) - temp * float(entropy.fermi_dirac(f))
+  view = {k: v.detach().clone().requires_grad_() for k, v in flat.items()}
+  f = occ_of(view)
+  plan.set_occupation(f.detach())
+  plan.set_band_energies(True)
+  plan.evaluate(params["w_re"], params["w_im"], grad=True)
+  sur = (plan.band_energies() * f).sum() - temp * entropy.fermi_dirac(f)
+  grads = torch.autograd.grad(sur, list(view.values()))
+  d = {k: torch.tensor(rng.standard_normal(tuple(v.shape)), device=v.device)
+       for k, v in flat.items()}
+  analytic = sum(float((g * d[k]).sum()) for k, g in zip(view, grads))
+  h = 1e-5
+  fp = free({k: v + h * d[k] for k, v in flat.items()})
+  fm = free({k: v - h * d[k] for k, v in flat.items()})
+  fd = (fp - fm) / (2 * h)
+  print(method, analytic, fd)
+  assert abs(analytic - fd) < 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("method", ["simplex-projector", "idempotent"])
+def test_cli_parameterised_occupation_run(tmp_path, method):
+  from jrystal_amd import config
+  from jrystal_amd.calc.energy_all_electrons import calc_with_output
+  p = tmp_path / "c.yaml"
+  p.write_text("crystal: si\ngrid_sizes: 16\nk_grid_sizes: 1\n"
+               "cutoff_energy: 10\nempty_bands: 2\nepoch: 40\n"
+               f"verbose: false\nseed: 1\noccupation: {method}\n"
+               "smearing: 0.001\n")
+  out = calc_with_output(config.get_config(str(p)))
+  assert out.steps == 40
+  e = np.asarray(out.total_energy_history)
+  assert np.all(np.isfinite(e)) and e[-1] < e[0]
+  rho = out.density.cpu().numpy()
+  # 28 electrons in 16 bands, whatever the occupation parameters
+  np.testing.assert_allclose(rho.sum() * out.crystal.vol / rho[0].size, 28.0,
+                             rtol=1e-10)

@@ -70,8 +70,14 @@ def test_driver_rejects_unsupported_options():
   from jrystal_amd import config
   from jrystal_amd.calc.energy_all_electrons import calc_with_output
   cfg = config.get_config()
+  cfg.occupation = "fermi"
+  with pytest.raises(ValueError):
+    calc_with_output(cfg)
+  cfg = config.get_config()
   cfg.occupation = "simplex-projector"
-  with pytest.raises(NotImplementedError):
+  cfg.optimizer = "cg"
+  cfg.smearing = 0.0
+  with pytest.raises(NotImplementedError):  # cg keeps occupations fixed
     calc_with_output(cfg)
 
 
