@@ -67,16 +67,32 @@ def algorithmic_bytes(cls, plan, chunks):
   return None
 
 
+def cpu_host():
+  """Host facts the CPU figure depends on."""
+  model = None
+  try:
+    with open("/proc/cpuinfo") as f:
+      model = next((ln.split(":", 1)[1].strip() for ln in f
+                    if ln.startswith("model name")), None)
+  except OSError:
+    pass
+  return {"nproc": os.cpu_count(), "cpu_model": model,
+          "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(wl, params, target_s, nthreads):
   """Oracle (NumPy/SciPy restatement of the reference) on a k-point subset
-  of the same workload, scaled to whole evaluations per second."""
+  of the same workload, scaled to whole evaluations per second; FFT
+  workers and BLAS threads both set to nthreads."""
   sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
   import numpy as np
+  from threadpoolctl import threadpool_limits
   from oracle import adjoint as OA
   from oracle import grid as OG
   from oracle import occupation as OO
   from oracle import pw as OP
   OP.set_workers(nthreads)
+  limits = threadpool_limits(limits=nthreads)
   cr = wl.crystal
   nk_full = wl.kpts.shape[0]
 
@@ -103,6 +119,7 @@ def cpu_baseline(wl, params, target_s, nthreads):
   t1 = run(1)
   nk_s = int(max(1, min(nk_full, round(target_s / max(t1, 1e-3)))))
   t = run(nk_s)
+  limits.unregister()
   evals_per_s = nk_s / (t * nk_full)
   return {"value": evals_per_s, "unit": "energy+gradient evaluations/s",
           "cores": nthreads, "kind": "port",
@@ -110,7 +127,7 @@ def cpu_baseline(wl, params, target_s, nthreads):
                      f"restricted to {nk_s} of {nk_full} k-points "
                      f"({t:.2f} s), scaled by {nk_full}/{nk_s}; oracle/ "
                      f"NumPy+SciPy FP64 restatement, scipy.fft workers="
-                     f"{nthreads}")}
+                     f"{nthreads}, BLAS threads {nthreads}")}
 
 
 def time_to(history, e_star, tol):
@@ -385,6 +402,11 @@ def main():
   if not args.no_cpu_baseline:
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     cpu = cpu_baseline(wl, params, args.cpu_seconds, nthreads)
+    cpu["host"] = cpu_host()
+    # the reference's own thread setting (opt_utils.py:45-47)
+    c4 = cpu_baseline(wl, params, args.cpu_seconds / 2, 4)
+    cpu["value_4_threads"] = c4["value"]
+    cpu["sample_4_threads"] = c4["sample"]
     if conv is not None:
       # CPU seconds per evaluation of the convergence cell, times the
       # evaluations the plain method needed on the GPU
