@@ -160,6 +160,15 @@ def convergence(time_cap, tol=1e-6):
   e_best = None
   p = wl.params(0)
   start = {k: torch.tensor(p[k], device=plan.device) for k in ("w_re", "w_im")}
+  # untimed first use of every kernel and torch operation the runs share
+  # (the first run would otherwise pay the loading for all of them)
+  plan.evaluate(start["w_re"], start["w_im"], grad=True)
+  warm = learned.LearnedFactor(plan, net)
+  learned.density_stats(plan, warm.stats)
+  learned.apply(plan, net, warm.stats,
+                gap=learned.band_gaps(plan) if net.use_gap else None,
+                factor=warm.factor)
+  torch.cuda.synchronize()
   for mode in ("learned", "learned_warm", "plain"):
     print(f"bench: convergence run {mode}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
