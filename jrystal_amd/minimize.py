@@ -66,6 +66,12 @@ class MinimizeResult:
 
 
 class Minimizer:
+  """Preconditioned CG on one plan from params {'w_re', 'w_im'} (copied to
+  the device).  factor_fn(minimizer) returns the per-coefficient factor of
+  the preconditioner (or None for plain TPA); restart is the gauge-reset
+  interval and first_restart the length of the first cycle; line_search is
+  "parabola" (default) or "secant".  run() minimises; params() returns the
+  current W."""
 
   def __init__(self, plan: EnergyPlan, params, factor_fn: Optional[Callable]
                = None, restart: int = PLAIN_RESTART, step0: float = 0.5,
