@@ -214,10 +214,11 @@ int jrx_mlp_mse_loss(const float* y, const float* target, int64_t n,
 int jrx_mlp_adam(float* params, const float* grad, float* m, float* v,
                  int64_t n, double lr, double b1, double b2, double eps,
                  int64_t count, void* stream);
-/* One pass of the CG update: h1 = K * factor * g1 (as
- * jrx_plan_precondition) and out[0..2] = (g1.h1, g0.h1, g1.d) summed over
- * the real and imaginary planes (deterministic).  scratch: device buffer
- * of jrx_cg_dots_workspace_bytes(). */
+/* Minimiser support (no counterpart in the reference): one pass of the CG
+ * update, h1 = K * factor * g1 (as jrx_plan_precondition) and
+ * out[0..2] = (g1.h1, g0.h1, g1.d) summed over the real and imaginary
+ * planes (deterministic).  scratch: device buffer of
+ * jrx_cg_dots_workspace_bytes(). */
 size_t jrx_cg_dots_workspace_bytes(void);
 int jrx_plan_precondition_dots(jrx_plan* plan, const double* g1_re,
                                const double* g1_im, const double* g0_re,
