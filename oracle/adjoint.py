@@ -151,6 +151,40 @@ def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
   return parts, grad, rho
 
 
+def band_energies(params, prob: Problem, k_chunk: Optional[int] = None):
+  """dE/df[s,k,b] for fixed W: 1/2 sum_G |G+k|^2 |Q_Gb|^2 (kinetic) plus
+  sum_r v_s(r) |psi_b(r)|^2 with v = dE/drho of potential_real (Hartree,
+  external and xc, the Omega/N weight included), for every band whether
+  occupied or not.  Real-space form: independent of the reciprocal-space
+  gather that the gradient uses."""
+  w = params["w_re"] + 1j * params["w_im"]
+  q = np.linalg.qr(w, mode="reduced")[0]
+  ns, nk, ng, nb = q.shape
+  mask = prob.mask
+  n = int(np.prod(mask.shape))
+  scale = n / np.sqrt(prob.vol)
+  k_chunk = k_chunk or max(1, min(nk, 8))
+  occ = prob.occupation
+  rho = np.zeros((ns,) + mask.shape)
+  psis = []
+  for s in range(ns):
+    for k0 in range(0, nk, k_chunk):
+      k1 = min(nk, k0 + k_chunk)
+      box = _pw.expand_coefficient(q[s, k0:k1], mask)
+      psi = sfft.ifftn(box, axes=(-3, -2, -1), workers=_pw._WORKERS) * scale
+      a2 = np.real(np.conj(psi) * psi)
+      rho[s] += np.einsum("kbxyz,kb->xyz", a2, occ[s, k0:k1])
+      psis.append((s, k0, k1, a2))
+  v = potential_real(rho, prob)[0]
+  out = np.empty((ns, nk, nb))
+  for s, k0, k1, a2 in psis:
+    out[s, k0:k1] = np.einsum("xyz,kbxyz->kb", v[s], a2)
+  gm = prob.mask_g()
+  gk2 = np.sum((gm[None] + prob.kpts[:, None, :])**2, axis=-1)
+  out += 0.5 * np.einsum("kg,skgb->skb", gk2, np.real(np.conj(q) * q))
+  return out
+
+
 def total_energy_literal(params, prob: Problem):
   """The driver closure calc_ground_state_energy_all_electrons.py:119-137
   composed from the reference's public functions (full-box layout)."""

@@ -107,3 +107,39 @@ def test_chunked_adjoint_equals_reference_composition():
   b = OA.total_energy_literal(params, prob)
   for k in b:
     np.testing.assert_allclose(a[k], b[k], rtol=1e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("make", [tiny])
+def test_band_energies_are_the_occupation_derivative(make):
+  """oracle.band_energies (real-space <psi|H|psi>) against central
+  differences of the oracle energy in each occupation number, including a
+  band with zero occupation."""
+  import dataclasses
+  _, prob, params = make()
+  rng = np.random.default_rng(21)
+  ns, nk, nb = prob.occupation.shape
+  f = rng.uniform(0.1, 2.0 / nk, (ns, nk, nb))
+  f[0, 0, nb - 1] = 0.0
+  pf = dataclasses.replace(prob, occupation=f)
+  eb = OA.band_energies(params, pf)
+  h = 1e-5
+  for idx in [(0, 0, 0), (0, nk - 1, nb - 1), (0, 0, nb - 1)]:
+    e = np.zeros_like(f)
+    e[idx] = h
+    ep = OA.energy_and_grad(params, dataclasses.replace(prob, occupation=f + e),
+                            want_grad=False)[0]["total"]
+    em = OA.energy_and_grad(params, dataclasses.replace(prob, occupation=f - e),
+                            want_grad=False)[0]["total"]
+    fd = (ep - em) / (2 * h)
+    assert abs(eb[idx] - fd) < 1e-7 * max(1.0, abs(fd)), (idx, eb[idx], fd)
+
+
+def test_band_energies_match_the_gather_form():
+  """sum_b f_b e_b equals Re <Q, Qbar> / 2 (the reciprocal-space form the
+  gradient uses: Qbar = f (2/sqrt(vol) FFT(v psi) + |G+k|^2 Q))."""
+  _, prob, params = tiny()
+  parts, _, _, ex = OA.energy_and_grad(params, prob, extras=True)
+  eb = OA.band_energies(params, prob)
+  lhs = np.sum(prob.occupation * eb)
+  rhs = 0.5 * np.real(np.sum(np.conj(ex["q"]) * ex["qbar"]))
+  assert abs(lhs - rhs) < 1e-10 * max(1.0, abs(rhs))
