@@ -132,3 +132,21 @@ def test_entropy_torch_value_and_gradient():
   e[0, 1, 2] = h
   fd = (OO.fermi_dirac_entropy(occ + e) - OO.fermi_dirac_entropy(occ - e)) / (2 * h)
   assert abs(OO.fermi_dirac_entropy_grad(occ)[0, 1, 2] - fd) < 1e-6
+
+
+@pytest.mark.parametrize("x,total,expect", [
+  ([0.3], 1.0, [1.0]),
+  ([0.2, 0.2, 0.2], 1.5, [0.5, 0.5, 0.5]),
+  # total = n (no empty bands): the reference's shift rule divides 0/0 at
+  # the last sorted entry, never selects k = n and falls back to the
+  # uniform shift, so the sum stays below the target (reproduced quirk)
+  ([0.9, 0.1, 0.5], 3.0, [1.0, 0.6, 1.0]),
+  # total = 0: likewise the uniform shift, sum above the target
+  ([0.9, 0.1, 0.5], 0.0, [0.4, 0.0, 0.0]),
+])
+def test_proj_edge_cases_follow_the_reference_rule(x, total, expect):
+  from jrystal_amd import occupation as O
+  a = OO.proj(np.array(x), total)[0]
+  b = O.proj(torch.tensor(x, dtype=torch.float64), total).numpy()
+  np.testing.assert_allclose(a, expect, atol=1e-15)
+  np.testing.assert_allclose(b, expect, atol=1e-15)
