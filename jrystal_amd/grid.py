@@ -99,10 +99,9 @@ def estimate_max_cutoff_energy(cell_vectors, mask):
 
 
 def translation_vectors(cell_vectors, cutoff=1e4):
-  """grid.py:213-236.  Like the reference, the grid is built with
-  normalize=True (fftfreq(n, 1) = m/n), i.e. it holds fractions (m/n) a_i of
-  the lattice vectors; kept as is so the nuclear-repulsion constant matches
-  the reference on the same inputs."""
+  """grid.py:213-236: lattice translations m_1 a_1 + m_2 a_2 + m_3 a_3 with
+  m_i = fftfreq(n, 1/n) (integers -n/2 .. n/2-1), n = ceil(cutoff/|sum a|^2);
+  the reference calls _vector_grid with its default normalize=False."""
   cell_vectors = np.asarray(cell_vectors, np.float64)
   n = int(np.ceil(cutoff / np.linalg.norm(np.sum(cell_vectors, axis=0))**2))
-  return _vector_grid(cell_vectors, [n] * 3, normalize=True).reshape(-1, 3)
+  return _vector_grid(cell_vectors, [n] * 3).reshape(-1, 3)

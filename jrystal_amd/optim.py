@@ -46,7 +46,12 @@ class Adam:
 
 def exponential_decay(init_value, transition_steps, decay_rate,
                       end_value=None):
-  """optax.exponential_decay with transition_begin = 0, staircase = False."""
+  """optax.exponential_decay with transition_begin = 0, staircase = False.
+  Like optax, a non-positive transition_steps or a zero decay_rate gives the
+  constant schedule init_value (the driver passes epoch // 2, which is 0 for
+  epoch < 2)."""
+  if transition_steps <= 0 or decay_rate == 0:
+    return lambda count: init_value
 
   def schedule(count):
     v = init_value * decay_rate**(count / transition_steps)

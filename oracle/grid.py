@@ -111,16 +111,12 @@ def estimate_max_cutoff_energy(cell_vectors, mask):
 
 
 def translation_vectors(cell_vectors, cutoff=1e4):
-  """grid.py:213-236, restated literally.
-
-  NOTE: the reference builds this grid with ``_vector_grid(...,
-  normalize=True)``, i.e. fftfreq(n, 1) = m/n, so the "translations" are
-  fractions (m/n) a_i of the lattice vectors, not lattice translations.  The
-  oracle keeps that behaviour because the reported nuclear-repulsion constant
-  must match the reference on the same inputs.
-  """
+  """grid.py:213-236: ``_vector_grid(cell_vectors, [n]*dim)`` with the
+  default normalize=False (grid.py:96), i.e. fftfreq(n, 1/n) = integers m, so
+  the rows are the lattice translations sum_i m_i a_i, m_i in -n/2..n/2-1,
+  with n = ceil(cutoff / |a_1 + a_2 + a_3|^2)."""
   cell_vectors = np.asarray(cell_vectors, np.float64)
   dim = cell_vectors.shape[0]
   n = int(np.ceil(cutoff / np.linalg.norm(np.sum(cell_vectors, axis=0))**2))
-  grid = _vector_grid(cell_vectors, [n] * dim, normalize=True)
+  grid = _vector_grid(cell_vectors, [n] * dim)
   return np.reshape(grid, [-1, dim])

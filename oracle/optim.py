@@ -36,7 +36,11 @@ class Adam:
 
 def exponential_decay(init_value, transition_steps, decay_rate,
                       end_value=None):
-  """optax.exponential_decay, transition_begin = 0, no staircase."""
+  """optax.exponential_decay, transition_begin = 0, no staircase; constant
+  init_value when transition_steps <= 0 or decay_rate == 0, as optax."""
+  if transition_steps <= 0 or decay_rate == 0:
+    return lambda count: init_value
+
   def f(count):
     v = init_value * decay_rate**(count / transition_steps)
     if end_value is not None:

@@ -38,6 +38,48 @@ def test_exponential_decay_matches_oracle():
   np.testing.assert_allclose(a(2500), 100.0 * 1e-5)
 
 
+def test_exponential_decay_non_positive_transition_is_constant():
+  """optax returns the constant init_value for transition_steps <= 0 (the
+  driver's epoch // 2 at epoch < 2) and for decay_rate == 0."""
+  from jrystal_amd.optim import exponential_decay
+  for ts, rate in ((0, 1e-5), (-3, 0.5), (10, 0.0)):
+    a = exponential_decay(100.0, ts, rate, 0.001)
+    b = OOpt.exponential_decay(100.0, ts, rate, 0.001)
+    for i in (0, 1, 7, 1000):
+      assert a(i) == 100.0 and b(i) == 100.0
+
+
+def test_oracle_adam_closed_forms():
+  """optax.adam (eps = 1e-8, eps_root = 0) in closed form: the bias
+  corrections make the first step -lr g/(|g| + eps) elementwise, and under a
+  constant gradient m_hat = g and v_hat = g^2 at every t, so every step
+  equals the first."""
+  rng = np.random.default_rng(5)
+  g = rng.standard_normal((3, 7)) * np.array([1e-3, 1.0, 50.0])[:, None]
+  p0 = {"w": rng.standard_normal((3, 7))}
+  lr, eps = 0.01, 1e-8
+  for b1, b2 in ((0.9, 0.999), (0.9, 0.99), (0.5, 0.7)):
+    opt = OOpt.Adam(p0, learning_rate=lr, b1=b1, b2=b2)
+    p = opt.step(p0, {"w": g})
+    step1 = -lr * g / (np.abs(g) + eps)
+    np.testing.assert_allclose(p["w"] - p0["w"], step1, rtol=1e-12,
+                               atol=1e-15)
+    for _ in range(9):
+      q = opt.step(p, {"w": g})
+      np.testing.assert_allclose(q["w"] - p["w"], step1, rtol=1e-9,
+                                 atol=1e-15)
+      p = q
+  # a sign flip after one step: m_hat = (b1 (1-b1) g - (1-b1) g)/(1-b1^2)
+  opt = OOpt.Adam(p0, learning_rate=lr, b1=0.9, b2=0.999)
+  p = opt.step(p0, {"w": g})
+  q = opt.step(p, {"w": -g})
+  b1, b2 = 0.9, 0.999
+  mh = (b1 * (1 - b1) * g - (1 - b1) * g) / (1 - b1**2)
+  vh = (b2 * (1 - b2) * g**2 + (1 - b2) * g**2) / (1 - b2**2)
+  np.testing.assert_allclose(q["w"] - p["w"], -lr * mh / (np.sqrt(vh) + eps),
+                             rtol=1e-12, atol=1e-15)
+
+
 def test_convergence_checker():
   from jrystal_amd.optim import ConvergenceChecker
   c = ConvergenceChecker(3, 1e-3)

@@ -114,14 +114,24 @@ def test_cubic_mask_frequencies():
     assert kept == set(range(lo, hi + 1))
 
 
-def test_translation_vectors_literal_fractional_grid():
-  """grid.py:213-236 builds fractions (m/n) a_i, n = ceil(cutoff/|sum a|^2)."""
+def test_translation_vectors_integer_lattice_grid():
+  """grid.py:213-236 calls _vector_grid with its default normalize=False, so
+  the rows are integer combinations m_i a_i, m_i in fftfreq(n, 1/n), with
+  n = ceil(cutoff/|sum a|^2)."""
   a = 10.26 * np.eye(3)
   t = OG.translation_vectors(a, 2e4)
   n = int(np.ceil(2e4 / (3 * 10.26**2)))
   assert t.shape == (n**3, 3)
-  np.testing.assert_allclose(t[1], [0, 0, 10.26 / n])
-  np.testing.assert_allclose(t[n * n], [10.26 / n, 0, 0])
+  np.testing.assert_allclose(t[1], [0, 0, 10.26])
+  np.testing.assert_allclose(t[n * n], [10.26, 0, 0])
+  np.testing.assert_allclose(t[-1], [-10.26, -10.26, -10.26])
+  m = t / 10.26
+  np.testing.assert_allclose(m, np.round(m), atol=1e-12)
+  assert m.min() == -(n // 2) and m.max() == (n - 1) // 2
+  fcc = 5.13 * (np.ones((3, 3)) - np.eye(3))
+  tf = OG.translation_vectors(fcc, 2e4)
+  frac = tf @ np.linalg.inv(fcc)
+  np.testing.assert_allclose(frac, np.round(frac), atol=1e-12)
 
 
 def test_product_setup_matches_oracle():

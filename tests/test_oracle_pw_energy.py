@@ -187,15 +187,16 @@ def test_ewald_formula_matches_textbook_on_lattice_translations():
 
 
 def test_ewald_reference_grid_value():
-  """With the reference's fractional translation grid the constant is not
-  the physical Ewald energy; the restatement reproduces it as written."""
+  """nuclear_repulsion with the reference's translation grid (integer lattice
+  translations, 64^3 of them for cutoff 2e4) and eta = 0.1 equals the
+  textbook Ewald energy of the Si2 cell at Z = 4."""
   cr = si2_fcc(4.0)
   g = OG.g_vectors(cr.cell_vectors, (32, 32, 32))
   e = OE.nuclear_repulsion(cr.positions, cr.charges, cr.cell_vectors, g,
                            cr.vol, 0.1, 2e4)
-  assert e > 1e6
   ref = OE.ewald_lattice(cr.positions, cr.charges, cr.cell_vectors)
   np.testing.assert_allclose(ref, -8.40046478618, rtol=1e-10)
+  np.testing.assert_allclose(e, ref, atol=1e-9)
 
 
 def test_total_energy_public_composite():
