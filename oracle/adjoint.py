@@ -102,9 +102,13 @@ def qr_backward(q, r, qbar):
 
 
 def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
-                    want_grad: bool = True, extras: bool = False):
+                    want_grad: bool = True, extras: bool = False,
+                    grad_kpts=None):
   """Returns (energies dict incl. 'total', grad dict or None, rho); with
-  extras, a fourth item {'q': Q, 'qbar': Qbar} (both (s,k,g,b))."""
+  extras, a fourth item {'q': Q, 'qbar': Qbar} (both (s,k,g,b)).
+  grad_kpts (optional sequence of k indices) restricts the gradient to those
+  k-points (the density and energies still use every k); the other k-points
+  of the returned gradient are NaN."""
   w = params["w_re"] + 1j * params["w_im"]
   q, r = np.linalg.qr(w, mode="reduced")  # (s,k,g,b), (s,k,b,b)
   ns, nk, ng, nb = q.shape
@@ -133,18 +137,24 @@ def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
                     parts["xc"])
   if not want_grad:
     return parts, None, rho
-  qbar = np.empty_like(q)
+  qbar = np.full_like(q, np.nan)
+  if grad_kpts is None:
+    blocks = [np.arange(k0, min(nk, k0 + k_chunk))
+              for k0 in range(0, nk, k_chunk)]
+  else:
+    ks = np.unique(np.asarray(grad_kpts, int))
+    blocks = [ks[i:i + k_chunk] for i in range(0, len(ks), k_chunk)]
   for s in range(ns):
-    for k0 in range(0, nk, k_chunk):
-      k1 = min(nk, k0 + k_chunk)
-      box = _pw.expand_coefficient(q[s, k0:k1], mask)
+    for kb in blocks:
+      box = _pw.expand_coefficient(q[s, kb], mask)
       psi = sfft.ifftn(box, axes=(-3, -2, -1), workers=_pw._WORKERS) * scale
       hv = sfft.fftn(v[s] * psi, axes=(-3, -2, -1), workers=_pw._WORKERS)
       hv = _pw.squeeze_coefficient(hv, mask)  # (kc,g,b)
-      f = occ[s, k0:k1][:, None, :]
-      qbar[s, k0:k1] = f * (2.0 / np.sqrt(vol) * hv +
-                            gk2[k0:k1, :, None] * q[s, k0:k1])
-  wbar = qr_backward(q, r, qbar)
+      f = occ[s, kb][:, None, :]
+      qbar[s, kb] = f * (2.0 / np.sqrt(vol) * hv + gk2[kb, :, None] * q[s, kb])
+  wbar = np.full_like(q, np.nan)
+  kall = np.concatenate(blocks)
+  wbar[:, kall] = qr_backward(q[:, kall], r[:, kall], qbar[:, kall])
   grad = {"w_re": np.real(wbar), "w_im": np.imag(wbar)}
   if extras:
     return parts, grad, rho, {"q": q, "qbar": qbar}
