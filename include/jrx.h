@@ -106,6 +106,25 @@ int jrx_energy_grad(jrx_plan* plan, const double* w_re, const double* w_im,
                     double* rho, double* q, void* workspace,
                     size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference.  jrx_energy_grad for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels, with the workspace
+ * taken from a static device array of JRX_STATIC_WORKSPACE_BYTES that each
+ * GPU the library runs on holds until the process ends.  The call allocates
+ * nothing; the plan itself is still made by jrx_plan_create.  Plans whose
+ * jrx_plan_workspace_bytes exceeds the limit are rejected with
+ * JRX_ERR_UNSUPPORTED (the 2-atom 32^3 / 8 k / 8-band cell needs about
+ * 23 MB).  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (two streams, two host threads) race, and the array keeps the
+ * last call's contents, so the plan-level readers of a workspace
+ * (jrx_plan_band_norms, jrx_plan_precondition, ...) do not apply to it.
+ * Per-kernel timing (jrx_plan_set_timing) is not recorded by this call. */
+#define JRX_STATIC_WORKSPACE_BYTES ((size_t)64 << 20)
+int jrx_energy_grad_static(jrx_plan* plan, const double* w_re,
+                           const double* w_im, double* energies,
+                           double* dw_re, double* dw_im, double* rho,
+                           double* q, void* stream);
+
 /* Preconditioned minimisation support (no counterpart in the reference,
  * whose driver runs optax Adam on the raw parameters).  After
  * jrx_energy_grad the workspace holds, per state, tb = sum_G |G+k|^2 |q|^2
@@ -217,7 +236,7 @@ int jrx_mlp_adam(float* params, const float* grad, float* m, float* v,
 /* Minimiser support (no counterpart in the reference): one pass of the CG
  * update, h1 = K * factor * g1 (as jrx_plan_precondition) and
  * out[0..2] = (g1.h1, g0.h1, g1.d) summed over the real and imaginary
- * planes (deterministic).  scratch: device buffer of
+ * planes (deterministic).  dots_ws: device buffer of
  * jrx_cg_dots_workspace_bytes(). */
 size_t jrx_cg_dots_workspace_bytes(void);
 int jrx_plan_precondition_dots(jrx_plan* plan, const double* g1_re,
@@ -225,7 +244,7 @@ int jrx_plan_precondition_dots(jrx_plan* plan, const double* g1_re,
                                const double* g0_im, const double* d_re,
                                const double* d_im, const double* factor,
                                double* h1_re, double* h1_im, double* out,
-                               void* scratch, size_t scratch_bytes,
+                               void* dots_ws, size_t dots_ws_bytes,
                                const void* workspace, size_t workspace_bytes,
                                void* stream);
 /* Flat float64 vector helpers: out[0] = sum a*b (deterministic, device

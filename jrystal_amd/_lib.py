@@ -14,6 +14,8 @@ JRX_ERR_INVALID = 1
 JRX_ERR_UNSUPPORTED = 2
 JRX_ERR_HIP = 3
 JRX_ERR_NOMEM = 4
+# Size of the static device workspace of jrx_energy_grad_static (jrx.h).
+STATIC_WORKSPACE_BYTES = 64 << 20
 
 _lib = None
 
@@ -82,6 +84,7 @@ SIGNATURES = {
   "jrx_batch_capture": [_P, _P],
   "jrx_batch_replay": [_P, _P],
   "jrx_energy_grad": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_energy_grad_static": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
   "jrx_qr_workspace_bytes": [_I64, _I, _I],
   "jrx_qr": [_P, _P, _I64, _I, _I, _P, _P, _P, _SZ, _P],
   "jrx_qr_backward": [_P, _P, _P, _I64, _I, _I, _P, _P, _P, _SZ, _P],

@@ -65,6 +65,33 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // Blocks of the density-statistics reduction.
 constexpr int kStatBlocks = 256;
 
+// Workspace of jrx_energy_grad_static: a static device array, one copy per
+// GPU, held until the process ends.  In host code `scratch` names a host
+// stand-in, not the device array: kernels and copies get the address that
+// hipGetSymbolAddress returns on each device (static_workspace below).
+static __device__ __align__(256) char scratch[JRX_STATIC_WORKSPACE_BYTES];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, void*> g_scratch_addr;  // device -> address of scratch
+
+// Address of `scratch` on the current device, looked up once per device
+// (called by jrx_plan_create with the plan's device current).
+static int static_workspace(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void* ptr = nullptr;
+    JRX_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr || ((uintptr_t)ptr & 255) != 0) {
+      set_error("static workspace has no 256-byte aligned device address");
+      return JRX_ERR_HIP;
+    }
+    it = g_scratch_addr.emplace(device, ptr).first;
+  }
+  *out = it->second;
+  return JRX_OK;
+}
+
 }  // namespace jrx
 
 using namespace jrx;
@@ -89,6 +116,7 @@ struct jrx_plan {
   int xline_blocks = 0;  // target workgroups of an x-line pass (0 = 4096)
   bool qr_adaptive = true;  // one CholeskyQR pass where W is well conditioned
   bool band_e = false;      // energy+gradient evaluations also form dE/df
+  void* static_ws = nullptr;  // device address of `scratch` on this device
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
   // timing
@@ -163,6 +191,10 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     return JRX_ERR_UNSUPPORTED;
   }
   JRX_CHECK_HIP(hipGetDevice(&p->device));
+  {
+    int rc_ws = static_workspace(p->device, &p->static_ws);
+    if (rc_ws) return rc_ws;
+  }
   p->ns = d->ns; p->nk = d->nk; p->nb = d->nb;
   p->nx = d->nx; p->ny = d->ny; p->nz = d->nz; p->natom = d->natom;
   p->P = (int64_t)d->ns * d->nk;
@@ -559,14 +591,14 @@ extern "C" int jrx_plan_precondition_dots(
     jrx_plan* p, const double* g1_re, const double* g1_im,
     const double* g0_re, const double* g0_im, const double* d_re,
     const double* d_im, const double* factor, double* h1_re, double* h1_im,
-    double* out, void* scratch, size_t scratch_bytes, const void* workspace,
+    double* out, void* dots_ws, size_t dots_ws_bytes, const void* workspace,
     size_t workspace_bytes, void* stream) {
   JRX_REQUIRE(p && g1_re && g1_im && g0_re && g0_im && d_re && d_im &&
-                  h1_re && h1_im && out && scratch && workspace,
+                  h1_re && h1_im && out && dots_ws && workspace,
               "null argument");
   JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
-  JRX_REQUIRE(scratch_bytes >= jrx_cg_dots_workspace_bytes(),
-              "scratch too small (jrx_cg_dots_workspace_bytes)");
+  JRX_REQUIRE(dots_ws_bytes >= jrx_cg_dots_workspace_bytes(),
+              "dots workspace too small (jrx_cg_dots_workspace_bytes)");
   const int64_t n = p->S * p->ng;
   int64_t nblk = (n + 256 * 16 - 1) / (256 * 16);
   if (nblk < 1) nblk = 1;
@@ -574,7 +606,7 @@ extern "C" int jrx_plan_precondition_dots(
   const int64_t chunk = (n + nblk - 1) / nblk;
   const double* tb = reinterpret_cast<const double*>(
       static_cast<const char*>(workspace) + p->o_tb);
-  double* part = static_cast<double*>(scratch);
+  double* part = static_cast<double*>(dots_ws);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(tpa_dots_kernel, dim3((unsigned)nblk), dim3(256), 0, s,
                      g1_re, g1_im, g0_re, g0_im, d_re, d_im, tb, p->d_g_m,
@@ -937,9 +969,9 @@ int jrx_plan_read_timing(jrx_plan* plan, double* ms, int* launches) {
   return JRX_OK;
 }
 
-static int timing_begin(jrx_plan* p, hipStream_t s, int* slot) {
+static int timing_begin(jrx_plan* p, hipStream_t s, int* slot, bool on) {
   *slot = -1;
-  if (!p->timing) return JRX_OK;
+  if (!on || !p->timing) return JRX_OK;
   const int i = (int)p->records.size();
   while ((int)p->events.size() < 2 * (i + 1)) {
     hipEvent_t e;
@@ -958,10 +990,13 @@ static int timing_end(jrx_plan* p, hipStream_t s, int slot, int cls) {
   return JRX_OK;
 }
 
-int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
-                    double* energies, double* dw_re, double* dw_im,
-                    double* rho_out, double* q_out, void* workspace,
-                    size_t workspace_bytes, void* stream) {
+// One evaluation into the given workspace; `timed` allows the per-kernel
+// events of jrx_plan_set_timing (their first use creates HIP events).
+static int energy_grad_impl(jrx_plan* p, const double* w_re,
+                            const double* w_im, double* energies,
+                            double* dw_re, double* dw_im, double* rho_out,
+                            double* q_out, void* workspace,
+                            size_t workspace_bytes, void* stream, bool timed) {
   JRX_REQUIRE(p && w_re && w_im && energies, "null argument");
   JRX_REQUIRE((dw_re == nullptr) == (dw_im == nullptr),
               "dw_re and dw_im must both be given or both be NULL");
@@ -998,7 +1033,7 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   int slot;
 #define JRX_T(cls, x)                                   \
   do {                                                  \
-    if ((rc = timing_begin(p, s, &slot))) return rc;    \
+    if ((rc = timing_begin(p, s, &slot, timed))) return rc; \
     if ((rc = (x))) return rc;                          \
     if ((rc = timing_end(p, s, slot, cls))) return rc;  \
   } while (0)
@@ -1123,4 +1158,32 @@ int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
   return JRX_OK;
 }
 
+int jrx_energy_grad(jrx_plan* p, const double* w_re, const double* w_im,
+                    double* energies, double* dw_re, double* dw_im,
+                    double* rho_out, double* q_out, void* workspace,
+                    size_t workspace_bytes, void* stream) {
+  return energy_grad_impl(p, w_re, w_im, energies, dw_re, dw_im, rho_out,
+                          q_out, workspace, workspace_bytes, stream, true);
+}
+
+int jrx_energy_grad_static(jrx_plan* p, const double* w_re,
+                           const double* w_im, double* energies,
+                           double* dw_re, double* dw_im, double* rho_out,
+                           double* q_out, void* stream) {
+  JRX_REQUIRE(p, "null plan");
+  if (p->total > JRX_STATIC_WORKSPACE_BYTES) {
+    set_error("plan workspace of " + std::to_string(p->total) +
+              " bytes exceeds the static workspace of " +
+              std::to_string((size_t)JRX_STATIC_WORKSPACE_BYTES) +
+              " bytes (use jrx_energy_grad with a caller workspace)");
+    return JRX_ERR_UNSUPPORTED;
+  }
+  // The address was looked up when the plan was created, on its device.
+  JRX_REQUIRE(p->static_ws, "static workspace address unknown");
+  return energy_grad_impl(p, w_re, w_im, energies, dw_re, dw_im, rho_out,
+                          q_out, p->static_ws, JRX_STATIC_WORKSPACE_BYTES,
+                          stream, false);
+}
+
 }  // extern "C"
+
