@@ -12,8 +12,9 @@ def main(argv=None):
   parser.add_argument("-m", "--mode", choices=["energy", "band"],
                       default="energy")
   parser.add_argument("-c", "--config", default="config.yaml")
-  parser.add_argument("-l", "--load", help="(accepted; unused, as in the "
-                      "reference)")
+  parser.add_argument("-l", "--load",
+                      help="energy mode: resume from a checkpoint .npz that "
+                      "an earlier run wrote under its save_dir")
   args = parser.parse_args(argv)
   cfg = config.get_config(args.config)
   logging.basicConfig(level=logging.INFO if cfg.verbose else logging.WARNING,
@@ -22,7 +23,7 @@ def main(argv=None):
     if cfg.use_pseudopotential:
       calc.energy_normcons(cfg)
     else:
-      calc.energy_all_electrons(cfg)
+      calc.energy_all_electrons(cfg, resume=args.load)
   else:
     if cfg.use_pseudopotential:
       calc.band_normcons(cfg)

@@ -18,7 +18,7 @@ from typing import List
 import numpy as np
 import torch
 
-from .. import energy, entropy, grid, occupation, pw
+from .. import checkpoint, energy, entropy, grid, occupation, pw
 from ..config import JrystalConfigDict
 from ..crystal import Crystal
 from ..objective import ENERGY_KEYS, EnergyPlan
@@ -68,7 +68,35 @@ def create_freq_mask(config, crystal):
   raise ValueError("freq_mask_method must be either cubic or spherical.")
 
 
-def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
+def _checkpoint_path(config):
+  if not config.save_dir:
+    return None
+  return os.path.join(config.save_dir, "checkpoint.npz")
+
+
+def _load_resume(path, plan, occ_keys=()):
+  """A checkpoint of this driver, checked against the plan's shapes."""
+  ck = checkpoint.load(path)
+  for k in ("w_re", "w_im"):
+    if tuple(ck["params"][k].shape) != tuple(plan.param_shape):
+      raise ValueError(f"checkpoint {path}: {k} has shape "
+                       f"{ck['params'][k].shape}, the run needs "
+                       f"{plan.param_shape}")
+  missing = [k for k in occ_keys if k not in ck["occ_params"]]
+  if missing:
+    raise ValueError(f"checkpoint {path} lacks occupation parameters "
+                     f"{missing}")
+  return ck
+
+
+def calc_with_output(config: JrystalConfigDict,
+                     resume: str = None) -> GroundStateEnergyOutput:
+  """resume: a checkpoint written by an earlier run of the same config
+  (its save_dir); the run continues from its step with the same parameters,
+  occupation parameters, Adam moments and energy history, so an
+  interrupted and resumed Adam run ends bitwise equal to an uninterrupted
+  one.  The CG optimisers restart their search direction from the saved
+  parameters."""
   if config.use_pseudopotential:
     raise NotImplementedError("pseudopotential energies are outside the "
                               "all-electron path")
@@ -115,6 +143,10 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
                          spin_restricted=config.spin_restricted,
                          dist=config.param_dist)
   if config.optimizer != "adam":
+    if resume is not None:
+      ck = _load_resume(resume, plan)
+      params = {k: torch.as_tensor(v, device=plan.device)
+                for k, v in ck["params"].items()}
     return _calc_cg(config, cr, plan, params, occ, ew)
   opt_args = dict(config.optimizer_args)
   if parameterised:
@@ -126,6 +158,34 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   else:
     leaves = params
   opt = Adam(leaves, **opt_args)
+  history = []
+  step0 = 0
+  if resume is not None:
+    ck = _load_resume(resume, plan, occ_flat if parameterised else ())
+    if ck["adam"] is None:
+      raise ValueError(f"checkpoint {resume} holds no Adam state")
+    for k, v in leaves.items():
+      src = ck["params"][k] if k in ck["params"] else ck["occ_params"][k]
+      v.copy_(torch.as_tensor(src, device=v.device))
+      opt.m[k].copy_(torch.as_tensor(ck["adam"]["m"][k], device=v.device))
+      opt.v[k].copy_(torch.as_tensor(ck["adam"]["v"][k], device=v.device))
+    opt.count = ck["adam"]["count"]
+    history = [float(e) for e in ck["energy_history"]]
+    step0 = ck["step"]
+    log.info("resumed from %s at step %d", resume, step0)
+  ck_path = _checkpoint_path(config)
+  ck_every = int(config.checkpoint_every or 0)
+
+  def save_checkpoint(step):
+    occ_now = (get_occupation(occ_rebuild(occ_flat)).detach()
+               if parameterised else occ)
+    checkpoint.save(
+      ck_path, step=step, params=params, occupation=occ_now,
+      energy_history=history,
+      adam={"count": opt.count, "m": opt.m, "v": opt.v},
+      occ_params=occ_flat if parameterised else None,
+      meta={"crystal": str(config.crystal), "seed": int(config.seed),
+            "optimizer": "adam"})
   if config.smearing > 0.0:
     temperature = exponential_decay(100.0, config.epoch // 2,
                                     config.smearing / 100, config.smearing)
@@ -134,14 +194,14 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
       return 0.0
   checker = ConvergenceChecker(config.convergence_window_size,
                                config.convergence_condition)
+  checker.history = history[-checker.window_size:]
   log.info("crystal %s: %d electrons, %d bands, %d k-points, G_n=%d",
            cr.symbols, ne, num_bands, nk, plan.num_g)
-  history = []
   converged = False
   out = {}
   t0 = time.perf_counter()
-  i = 0
-  for i in range(config.epoch):
+  i = step0 - 1
+  for i in range(step0, config.epoch):
     temp = temperature(i)
     if parameterised:
       # F = E(W, f) - T S(f): dF/dW from the plan; dF/df = band energies
@@ -164,6 +224,8 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
     history.append(etot)
     if not np.isfinite(etot):
       raise FloatingPointError(f"energy became {etot} at step {i}")
+    if ck_path and ck_every > 0 and (i + 1) % ck_every == 0:
+      save_checkpoint(i + 1)
     if checker.check(etot):
       converged = True
       log.info("Converged.")
@@ -174,6 +236,8 @@ def calc_with_output(config: JrystalConfigDict) -> GroundStateEnergyOutput:
   seconds = time.perf_counter() - t0
   if not converged:
     log.warning("Did not converge.")
+  if ck_path:
+    save_checkpoint(i + 1)
   if parameterised:
     with torch.no_grad():
       plan.set_occupation(get_occupation(occ_rebuild(occ_flat)))
@@ -226,10 +290,17 @@ def _calc_cg(config, cr, plan, params, occ, ew):
            res.status, res.iterations, res.evaluations, res.seconds)
   history = [e for _, e in res.history]
   p = {"w_re": m.params()["w_re"].clone(), "w_im": m.params()["w_im"].clone()}
+  ck_path = _checkpoint_path(config)
+  if ck_path:
+    checkpoint.save(ck_path, step=res.iterations, params=p, occupation=occ,
+                    energy_history=history,
+                    meta={"crystal": str(config.crystal),
+                          "seed": int(config.seed),
+                          "optimizer": config.optimizer})
   return _final(config, cr, plan, p, occ, ew, history, converged,
                 res.iterations, res.seconds)
 
 
-def calc(config: JrystalConfigDict):
+def calc(config: JrystalConfigDict, resume: str = None):
   """Returns the density (spin, x, y, z), as the reference driver does."""
-  return calc_with_output(config).density
+  return calc_with_output(config, resume=resume).density

@@ -53,6 +53,8 @@ DEFAULT_CONFIG = {
   # Keys of this implementation (not in the reference):
   "param_dist": "uniform",   # U[0,1) as the reference; or "normal"
   "chunk_states": 0,         # FFT chunk of the fused evaluation
+  "checkpoint_every": 0,     # with save_dir: a checkpoint every N steps
+                             # (0: only at the end of the run)
 }
 
 

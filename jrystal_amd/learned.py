@@ -37,7 +37,7 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, checkpoint
 from .objective import EnergyPlan
 
 NFEAT = 8
@@ -274,14 +274,69 @@ def collect_trajectory(wl, seed: int, snap_every: int = 8,
                        samples: int = 512, ws_samples: int = 4096,
                        max_iter: int = 800, max_snaps: int = 60,
                        rng: Optional[np.random.Generator] = None,
-                       device=None) -> Dataset:
-  """One plain-method trajectory of workload wl from wl.params(seed), turned
-  into training samples of both heads."""
+                       device=None, out_dir: Optional[str] = None) -> Dataset:
+  """One plain-method trajectory of workload wl from wl.params(seed), stored
+  as checkpoint files (jrystal_amd.checkpoint: the start, every snap_every-th
+  iterate, the converged end) in out_dir (a temporary directory when None)
+  and turned into training samples of both heads by trajectory_dataset."""
+  import tempfile
   from .minimize import Minimizer
-  rng = rng if rng is not None else np.random.default_rng(seed)
+  tmp = None
+  if out_dir is None:
+    tmp = tempfile.TemporaryDirectory(prefix="jrx_traj_")
+    out_dir = tmp.name
+  try:
+    plan = EnergyPlan.from_workload(wl, device=device)
+    meta = {"workload": wl.name, "seed": int(seed)}
+    p0 = wl.params(seed)
+    start = checkpoint.save(os.path.join(out_dir, "start.npz"), step=0,
+                            params=p0, occupation=plan.occupation,
+                            meta=dict(meta, kind="start"))
+    snaps = []
+
+    def cb(m, it, e):
+      if it % snap_every or len(snaps) >= max_snaps:
+        return
+      snaps.append(checkpoint.save(
+        os.path.join(out_dir, f"snap_{it:05d}.npz"), step=it,
+        params={"w_re": m.W[0], "w_im": m.W[1]}, occupation=plan.occupation,
+        energy_history=[e], meta=dict(meta, kind="snapshot")))
+
+    m = Minimizer(plan, p0)
+    r = m.run(max_iter=max_iter, callback=cb)
+    final = checkpoint.save(
+      os.path.join(out_dir, "final.npz"), step=r.iterations,
+      params={"w_re": m.W[0], "w_im": m.W[1]}, occupation=plan.occupation,
+      energy_history=[e for _, e in r.history],
+      meta=dict(meta, kind="final", status=r.status))
+    plan.close()
+    return trajectory_dataset(wl, start, snaps, final, samples=samples,
+                              ws_samples=ws_samples,
+                              rng=rng if rng is not None
+                              else np.random.default_rng(seed),
+                              device=device)
+  finally:
+    if tmp is not None:
+      tmp.cleanup()
+
+
+def trajectory_dataset(wl, start: str, snapshots: List[str], final: str,
+                       samples: int = 512, ws_samples: int = 4096,
+                       rng: Optional[np.random.Generator] = None,
+                       device=None) -> Dataset:
+  """Training samples from a trajectory stored as checkpoint files.
+
+  Preconditioner head, per snapshot W_t (samples random occupied
+  coefficients): features at W_t, a = Re(conj(D) P_TPA g) and
+  b = |P_TPA g|^2 with g = dE/dW at W_t and D the Grassmann direction from
+  span(Q_t) to the converged span(Q*), carried to W_t's gauge (D R_oo).
+  Warm-start head: features at the start, target 1/2 log of the converged
+  occupied-subspace weight of each plane wave."""
+  rng = rng if rng is not None else np.random.default_rng(0)
   plan = EnergyPlan.from_workload(wl, device=device)
   ns, nk, ng, nb = plan.param_shape
   nocc = _occupied_block(plan.occupation)
+  dev = plan.device
 
   def sample_idx(n):
     s = rng.integers(0, ns, n)
@@ -290,34 +345,35 @@ def collect_trajectory(wl, seed: int, snap_every: int = 8,
     b = rng.integers(0, nocc, n)
     return ((s * nk + k) * ng + g) * nb + b
 
-  p0 = wl.params(seed)
+  def params_of(path):
+    ck = checkpoint.load(path)
+    return {k: torch.as_tensor(v, device=dev)
+            for k, v in ck["params"].items()}
+
+  p0 = params_of(start)
   plan.evaluate(p0["w_re"], p0["w_im"], grad=False)
   idx_ws = sample_idx(ws_samples)
   x_ws = coefficient_features(plan, idx_ws, density_stats(plan)).cpu().numpy()
 
-  snaps = []
-
-  def cb(m, it, e):
-    if it % snap_every or len(snaps) >= max_snaps:
-      return
-    qr, qi = plan.orthonormal_params()
-    q = _host_complex(qr, qi)[..., :nocc]
-    w = _host_complex(m.W[0], m.W[1])[..., :nocc]
-    idx = sample_idx(samples)
-    h = (m.h[0].reshape(-1)[torch.as_tensor(idx, device=plan.device)].cpu()
-         .numpy() + 1j * m.h[1].reshape(-1)[torch.as_tensor(
-           idx, device=plan.device)].cpu().numpy())
-    x = coefficient_features(plan, idx, density_stats(plan),
-                             band_gaps(plan)).cpu().numpy()
-    snaps.append((q, w, idx, h, x))
-
-  m = Minimizer(plan, p0)
-  m.run(max_iter=max_iter, callback=cb)
+  pf = params_of(final)
+  plan.evaluate(pf["w_re"], pf["w_im"], grad=False)
   qr, qi = plan.orthonormal_params()
   qs = _host_complex(qr, qi)[..., :nocc]
 
   xs, as_, bs, ts = [], [], [], []
-  for q, w, idx, h, x in snaps:
+  for path in snapshots:
+    w_t = params_of(path)
+    out = plan.evaluate(w_t["w_re"], w_t["w_im"], grad=True)
+    hr, hi = plan.precondition(out["dw_re"], out["dw_im"])
+    qr, qi = plan.orthonormal_params()
+    q = _host_complex(qr, qi)[..., :nocc]
+    w = _host_complex(w_t["w_re"], w_t["w_im"])[..., :nocc]
+    idx = sample_idx(samples)
+    it = torch.as_tensor(idx, device=dev)
+    h = (hr.reshape(-1)[it].cpu().numpy() +
+         1j * hi.reshape(-1)[it].cpu().numpy())
+    x = coefficient_features(plan, idx, density_stats(plan),
+                             band_gaps(plan)).cpu().numpy()
     ov = np.linalg.svd(np.conj(np.swapaxes(q, -1, -2)) @ qs, compute_uv=False)
     if ov.min() < 0.2:  # far from the minimum: D is ill-defined
       continue
@@ -331,6 +387,7 @@ def collect_trajectory(wl, seed: int, snap_every: int = 8,
     as_.append(np.real(np.conj(t) * h))
     bs.append(np.abs(h)**2)
     ts.append(np.sum(np.abs(t)**2))
+  plan.close()
   wgt = np.sum(np.abs(qs)**2, axis=-1)                # (s,k,g)
   wnorm = wgt * ng / nocc
   sk_g = idx_ws // nb
