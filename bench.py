@@ -44,7 +44,11 @@ def algorithmic_bytes(cls, plan, chunks):
   N = nx * ny * nz
   P = plan.ns * plan.nk
   nxa = plan.active_planes
-  T = S * nxa * ny * nz * 16      # intermediate planes of all states
+  if getattr(plan, "row_path", False):
+    # row buffer R[state][z][row]: z-transformed mask rows of all states
+    T = S * nz * plan.num_rows * 16
+  else:
+    T = S * nxa * ny * nz * 16    # intermediate planes of all states
   Q = S * ng * 16                 # orthonormal coefficients
   W = 2 * S * ng * 8              # planar parameters (or gradient)
   rho = plan.ns * N * 8

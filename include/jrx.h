@@ -29,7 +29,15 @@ enum {
   JRX_ERR_NOMEM = 4
 };
 
-enum { JRX_XC_LDA_X = 0 };
+/* Exchange-correlation terms of jrx_problem.xc, OR-ed (the reference's
+ * '+'-joined xc strings, jrystal/_src/xc.py:222-253); 0 means lda_x.
+ * Terms other than lda_x need an unpolarised problem (ns = 1). */
+enum {
+  JRX_XC_LDA_X = 1,     /* Dirac exchange                                 */
+  JRX_XC_LDA_C_PW = 2,  /* Perdew-Wang 1992 correlation                   */
+  JRX_XC_GGA_X_PBE = 4, /* PBE exchange                                   */
+  JRX_XC_GGA_C_PBE = 8  /* PBE correlation                                */
+};
 
 const char* jrx_last_error(void);
 const char* jrx_version(void);
@@ -67,9 +75,13 @@ typedef struct {
   const double* positions; /* host [natom*3], bohr                          */
   const double* charges;   /* host [natom]                                  */
   const double* occupation;/* host [ns*nk*nb]                               */
-  int xc;                  /* JRX_XC_LDA_X                                  */
+  int xc;                  /* JRX_XC_* terms OR-ed; 0 = lda_x              */
   int chunk_states;        /* states per FFT chunk, 0 = automatic           */
-  int fft_variant;         /* 0 = automatic, 1 = runtime-radix kernels only */
+  int fft_variant;         /* 0 = automatic (row buffer for cubic 64^3 and
+                              128^3 boxes, else plane buffer), 1 = plane
+                              buffer with runtime-radix kernels only, 2 =
+                              plane buffer (register 64-point kernels where
+                              they apply)                                   */
   int xline_blocks;        /* target workgroups of an x-line pass, 0 = 4096 */
   int plane_pad;           /* extra elements per x-plane of the plane
                               buffer, -1 = automatic                        */
@@ -89,6 +101,8 @@ typedef struct {
   int chunk_states;   /* states per FFT chunk                               */
   int keep_planes;    /* 1: the adjoint sweep reuses the forward planes     */
   size_t workspace_bytes;
+  int row_path;       /* 1: row-buffer FFT path (cubic 64^3 / 128^3 boxes)  */
+  int rows;           /* (x, y) lines along z that hold mask points         */
 } jrx_plan_info;
 int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info);
 /* Number of float64 values in w_re (= in w_im): ns*nk*ng*nb. */

@@ -53,7 +53,8 @@ class JrxAdamConfig(ctypes.Structure):
 class JrxPlanInfo(ctypes.Structure):
   _fields_ = [("ng", ctypes.c_int), ("active_planes", ctypes.c_int),
               ("chunk_states", ctypes.c_int), ("keep_planes", ctypes.c_int),
-              ("workspace_bytes", ctypes.c_size_t)]
+              ("workspace_bytes", ctypes.c_size_t),
+              ("row_path", ctypes.c_int), ("rows", ctypes.c_int)]
 
 
 _P = ctypes.c_void_p

@@ -175,16 +175,6 @@ constexpr int kXBlock = 256;
 constexpr int kXCap = 2048;
 constexpr int kXEpt = kXCap / kXBlock;
 
-// Streaming access to the plane buffer (written once and read once per
-// sweep: gigabytes, far beyond the caches) carries the nontemporal hint.
-__device__ __forceinline__ void nt_store(double2* p, double2 v) {
-  __builtin_nontemporal_store(v.x, &p->x);
-  __builtin_nontemporal_store(v.y, &p->y);
-}
-__device__ __forceinline__ double2 nt_load(const double2* p) {
-  return make_double2(__builtin_nontemporal_load(&p->x),
-                      __builtin_nontemporal_load(&p->y));
-}
 
 // The x-line passes stage LZ lines of nx points in LDS, slot e = x * LZ + l
 // (line l = z - z0).  Thread t owns slots t + k * kXBlock; goff[k] is the

@@ -17,6 +17,17 @@ namespace jrx {
 
 constexpr int kMaxStages = 12;
 
+// Streaming access to the FFT intermediates (written once and read once per
+// sweep: gigabytes, far beyond the caches) carries the nontemporal hint.
+__device__ __forceinline__ void nt_store(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ double2 nt_load(const double2* p) {
+  return make_double2(__builtin_nontemporal_load(&p->x),
+                      __builtin_nontemporal_load(&p->y));
+}
+
 struct FftLen {
   int n;
   int nstages;
@@ -300,5 +311,90 @@ __device__ __forceinline__ void fft64_8x8(double2 (&v)[8], double2* base,
   wave_sync();
   Dft<8, INV>::run(v, nullptr, 0);
 }
+
+// 16-point DFT in registers: radix-2 split into two 8-point DFTs, with the
+// roots w16^t = tw[8t] taken from the length-128 table.
+template <bool INV>
+__device__ __forceinline__ void dft16(double2 (&u)[16],
+                                      const double2* __restrict__ tw128) {
+  double2 e[8], o[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    e[t] = u[2 * t];
+    o[t] = u[2 * t + 1];
+  }
+  Dft<8, INV>::run(e, nullptr, 0);
+  Dft<8, INV>::run(o, nullptr, 0);
+#pragma unroll
+  for (int t = 1; t < 8; ++t) o[t] = cmul(o[t], twid<INV>(tw128, 8 * t));
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    u[t] = cadd(e[t], o[t]);
+    u[t + 8] = csub(e[t], o[t]);
+  }
+}
+
+// 128 points on eight lanes: lane j holds x = j + 8m in v[m] (m = 0..15)
+// on entry and X[j + 8u] in v[u] on exit (the same layout), with
+//   Y_j[a] = sum_m x[j+8m] w16^{ma} (registers),  Y_j[a] *= w128^{ja},
+//   X[a + 16b] = sum_j Y_j[a] w8^{jb}  (lane a % 8, after an exchange).
+// The exchange goes through 64 LDS slots base[s * es] in two rounds
+// (a < 8, then a >= 8), so a line needs the same 64 slots as a 64-point
+// one.  All lanes of a line are in one wave.
+template <bool INV>
+__device__ __forceinline__ void fft128_16x8(double2 (&v)[16], double2* base,
+                                            int es, int j,
+                                            const double2* __restrict__ tw) {
+  dft16<INV>(v, tw);
+#pragma unroll
+  for (int a = 1; a < 16; ++a) v[a] = cmul(v[a], twid<INV>(tw, j * a));
+  double2 z0[8], z1[8];
+  wave_sync();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) base[(8 * a + ((j + a) & 7)) * es] = v[a];
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z0[i] = base[(8 * j + ((i + j) & 7)) * es];
+  wave_sync();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) base[(8 * a + ((j + a) & 7)) * es] = v[a + 8];
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z1[i] = base[(8 * j + ((i + j) & 7)) * es];
+  wave_sync();
+  Dft<8, INV>::run(z0, nullptr, 0);
+  Dft<8, INV>::run(z1, nullptr, 0);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    v[2 * b] = z0[b];
+    v[2 * b + 1] = z1[b];
+  }
+}
+
+// Register transform of one line of N points on eight lanes (N/8 values
+// per lane, element j + 8m of the line in v[m] on lane j, same layout on
+// exit); 64 exchange slots base[s * es].
+template <int N>
+struct RegFft;
+template <>
+struct RegFft<64> {
+  static constexpr int V = 8;
+  template <bool INV>
+  __device__ __forceinline__ static void run(double2 (&v)[8], double2* base,
+                                             int es, int j,
+                                             const double2* tw) {
+    fft64_8x8<INV>(v, base, es, j, tw);
+  }
+};
+template <>
+struct RegFft<128> {
+  static constexpr int V = 16;
+  template <bool INV>
+  __device__ __forceinline__ static void run(double2 (&v)[16], double2* base,
+                                             int es, int j,
+                                             const double2* tw) {
+    fft128_16x8<INV>(v, base, es, j, tw);
+  }
+};
 
 }  // namespace jrx

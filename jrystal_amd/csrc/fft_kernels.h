@@ -20,6 +20,16 @@ struct BoxGeom {
   int ps;               // element stride of the x-planes of the plane
                         // buffer T (ny*nz plus a pad that spreads the planes
                         // read together by the x-line passes over channels)
+  // Rows: the (x, y) lines along z that hold mask points, numbered in C
+  // order (x-major), so that the rows of active plane xa are
+  // row_off[xa] .. row_off[xa+1] and the mask points of a row are
+  // contiguous in g.
+  int nrows;
+  int rstride;          // elements per z-plane of one state in the row
+                        // buffer R[state][z][row] (nrows rounded up to 8)
+  const int* row_g0;    // [nrows+1] first mask point of each row
+  const int* g_row;     // [ng]      row of each mask point
+  const int* rowidx;    // [nxa][ny] row of (active plane, y), or -1
 };
 
 // y = scale * DFT_axis(x) along one axis of a (B, n0, n1, n2) complex box.
@@ -69,6 +79,7 @@ struct GatherParams {
   double B[9];         // reciprocal basis rows
   int nk, nb;          // state -> k index = (state / nb) % nk
   double two_over_vol;
+  const short* g_m;    // [ng*3] integer frequencies of the mask points
   // optional: pot_partial[state * nxa + plane] = Re sum_g conj(q_g) h_g over
   // the plane's mask points, h = FFT(v psi) (band energies)
   double* pot_partial;
@@ -97,6 +108,35 @@ int launch_xline_density64(const double2* T, double* rho_part, int groups,
 int launch_xline_vpsi64(double2* T, const double* v, int groups, int nstates,
                         const BoxGeom& geo, const double2* tw64,
                         hipStream_t s);
+
+// Row-buffer path (rowfft.hip) for cubic 64^3 and 128^3 boxes whose mask
+// spans at most 64 x-planes.  R[c][z][row] (row stride geo.rstride) holds
+// the z-transformed rows of chunk state c; densities and potentials on
+// this path are z-major, [z][y][x].  tw is the length-n twiddle table.
+bool use_rowpath(const BoxGeom& geo);
+// Row blocks of the row passes (the gather writes that many kinetic and
+// band-energy partials per state).
+int row_chunks(const BoxGeom& geo);
+// State groups of a z-plane pass.
+int zplane_groups(const BoxGeom& geo, int nstates);
+int launch_row_ifft(const double2* q, double2* R, int64_t state0, int nstates,
+                    const BoxGeom& geo, const double2* tw, hipStream_t s);
+// rho_part[g][z][y][x] += sum_{c in group g} occ |psi_c|^2 / vol
+int launch_zplane_density(const double2* R, double* rho_part, int groups,
+                          const double* occ, int64_t state0, int nstates,
+                          double inv_vol, const BoxGeom& geo,
+                          const double2* tw, hipStream_t s);
+// R[c] <- rows of FFT_xy(v * IFFT_xy(R[c])) (in place), v [z][y][x]
+int launch_zplane_vpsi(double2* R, const double* v, int groups, int nstates,
+                       const BoxGeom& geo, const double2* tw, hipStream_t s);
+// qbar and kinetic partials kin_partial[st * row_chunks + chunk]
+int launch_row_fft_gather(const double2* R, const double2* q, double2* qbar,
+                          double* kin_partial, int64_t state0, int nstates,
+                          const BoxGeom& geo, const double2* tw,
+                          const GatherParams& gp, hipStream_t s);
+// [ns][z][y][x] -> [ns][x][y][z] for cubic n^3 boxes.
+int launch_zmajor_to_xmajor(const double* in, double* out, int ns, int n,
+                            hipStream_t s);
 
 // Largest plane (ny*nz) the fused plane kernels hold in LDS.
 constexpr int kPlaneCap = 6400;

@@ -117,6 +117,10 @@ struct jrx_plan {
   bool qr_adaptive = true;  // one CholeskyQR pass where W is well conditioned
   bool band_e = false;      // energy+gradient evaluations also form dE/df
   void* static_ws = nullptr;  // device address of `scratch` on this device
+  int xc = JRX_XC_LDA_X;    // JRX_XC_* terms
+  bool rowpath = false;     // row-buffer FFT path (z-major rho and v)
+  int row_chunks = 0;       // row blocks of the row passes
+  CellParams cpz;           // the cell with box axes (z, y, x)
   GramSplit gsw, gsq;
   int kin_parts_max = 0, rec_parts = 0;
   // timing
@@ -124,6 +128,7 @@ struct jrx_plan {
   std::vector<hipEvent_t> events;
   std::vector<std::pair<int, int>> records;  // (class, event pair index)
   // workspace offsets (bytes)
+  size_t o_gb = 0;  // density-gradient boxes of GGA terms [3][N] complex
   size_t o_stats;
   size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_tb, o_S, o_R1, o_R2, o_Ri, o_A,
       o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, o_qrflag, o_bepart, o_pot,
@@ -181,15 +186,20 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
               "mask, kpts, cell and occupation are required");
   JRX_REQUIRE(d->natom >= 0 && (d->natom == 0 || (d->positions && d->charges)),
               "positions and charges are required");
-  JRX_REQUIRE(d->xc == JRX_XC_LDA_X, "only xc = lda_x is supported");
+  JRX_REQUIRE(d->xc >= 0 && d->xc <= 15,
+              "xc must be an OR of JRX_XC_* terms (0 = lda_x)");
+  p->xc = d->xc == 0 ? JRX_XC_LDA_X : d->xc;
+  if (p->xc != JRX_XC_LDA_X && d->ns != 1) {
+    set_error("xc terms other than lda_x need an unpolarised problem (ns = 1)");
+    return JRX_ERR_UNSUPPORTED;
+  }
   if (d->nb > kMaxBands) {
     set_error("nb above 64 is not supported by this build");
     return JRX_ERR_UNSUPPORTED;
   }
-  if ((int64_t)d->ny * d->nz > kPlaneCap) {
-    set_error("ny*nz above 6400 is not supported by this build");
-    return JRX_ERR_UNSUPPORTED;
-  }
+  JRX_REQUIRE(d->fft_variant >= 0 && d->fft_variant <= 2,
+              "fft_variant must be 0 (automatic), 1 (runtime radix) or 2 "
+              "(plane buffer)");
   JRX_CHECK_HIP(hipGetDevice(&p->device));
   {
     int rc_ws = static_workspace(p->device, &p->static_ws);
@@ -209,7 +219,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->cx = 0.75 * cbrt(3.0 / M_PI);
   // Geometry of the mask.
   const int nx = d->nx, ny = d->ny, nz = d->nz;
-  std::vector<int> plane_g0, plane_x, g_yz, row_off;
+  std::vector<int> plane_g0, plane_x, g_yz, row_off, row_g0, g_row;
   std::vector<short> rows, g_m;
   int ng = 0, max_rows = 0;
   auto fq = [](int j, int n) { return j < (n + 1) / 2 ? j : j - n; };
@@ -221,6 +231,8 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
       for (int z = 0; z < nz; ++z) {
         if (d->mask[((size_t)x * ny + y) * nz + z]) {
           if (first < 0) first = ng;
+          if (!any) row_g0.push_back(ng);  // first point of a new row
+          g_row.push_back((int)row_g0.size() - 1);
           g_yz.push_back(y * nz + z);
           g_m.push_back((short)fq(x, nx));
           g_m.push_back((short)fq(y, ny));
@@ -243,8 +255,15 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   JRX_REQUIRE(nx <= 2047 && ny <= 2047 && nz <= 2047, "grid too large");
   plane_g0.push_back(ng);
   row_off.push_back((int)rows.size());
+  row_g0.push_back(ng);
   std::vector<int> xmap(nx, -1);
   for (size_t i = 0; i < plane_x.size(); ++i) xmap[plane_x[i]] = (int)i;
+  // rows of the row-buffer path: rowidx[xa][y] = row of (plane xa, y)
+  const int nrows = (int)rows.size();
+  std::vector<int> rowidx(plane_x.size() * (size_t)ny, -1);
+  for (size_t xa = 0; xa < plane_x.size(); ++xa)
+    for (int r = row_off[xa]; r < row_off[xa + 1]; ++r)
+      rowidx[xa * ny + rows[r]] = r;
   p->ng = ng;
   p->geo.nx = nx; p->geo.ny = ny; p->geo.nz = nz; p->geo.ng = ng;
   p->geo.nxa = (int)plane_x.size();
@@ -265,6 +284,14 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   if ((rc = upload(p, g_m.data(), g_m.size(), &p->d_g_m))) return rc;
   if ((rc = upload(p, xmap.data(), xmap.size(), &tmp))) return rc;
   p->geo.xmap = tmp;
+  p->geo.nrows = nrows;
+  p->geo.rstride = (nrows + 7) / 8 * 8;
+  if ((rc = upload(p, row_g0.data(), row_g0.size(), &tmp))) return rc;
+  p->geo.row_g0 = tmp;
+  if ((rc = upload(p, g_row.data(), g_row.size(), &tmp))) return rc;
+  p->geo.g_row = tmp;
+  if ((rc = upload(p, rowidx.data(), rowidx.size(), &tmp))) return rc;
+  p->geo.rowidx = tmp;
   if ((rc = make_len(p, nx, &p->Lx))) return rc;
   if ((rc = make_len(p, ny, &p->Ly))) return rc;
   if ((rc = make_len(p, nz, &p->Lz))) return rc;
@@ -279,12 +306,28 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->cp.natom = d->natom;
   p->cp.pos = dpos;
   p->cp.charge = dchg;
+  // FFT path: the row buffer for cubic 64^3 / 128^3 boxes (fft_variant 0),
+  // else the plane buffer (register 64-point kernels where they apply, or
+  // the runtime-radix kernels everywhere with fft_variant 1).
+  // JRX_FFT_GENERIC=1 selects the runtime-radix kernels too (cross-checks).
+  const char* gen = getenv("JRX_FFT_GENERIC");
+  p->force_generic = (gen && gen[0] == '1') || d->fft_variant == 1;
+  p->rowpath = d->fft_variant == 0 && !p->force_generic &&
+               use_rowpath(p->geo);
+  if (!p->rowpath && (int64_t)ny * nz > kPlaneCap) {
+    set_error("ny*nz above 6400 is supported only on the row-buffer path "
+              "(cubic 64^3 or 128^3 boxes whose mask spans at most 64 "
+              "x-planes)");
+    return JRX_ERR_UNSUPPORTED;
+  }
   // LDS requirements against the device limit.
   int max_lds = 0;
   JRX_CHECK_HIP(hipDeviceGetAttribute(
       &max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
   const int nb = d->nb;
-  const size_t lds_plane = (size_t)ny * nz * sizeof(double2);
+  // (the row path's largest block holds a 128 x 66 z-plane)
+  const size_t lds_plane =
+      (p->rowpath ? (size_t)nz * 66 : (size_t)ny * nz) * sizeof(double2);
   const size_t nbp = 16 * (size_t)((nb + 15) / 16);
   const size_t lds_qr = 2 * nbp * nbp * sizeof(double2);
   const size_t lds_gram = 2 * nbp * nbp * sizeof(double2);  // pre + sums
@@ -309,7 +352,9 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     JRX_REQUIRE(ps < (int64_t)1 << 30, "plane too large");
     p->geo.ps = (int)ps;
   }
-  const size_t per_state = (size_t)p->geo.nxa * p->geo.ps * sizeof(double2);
+  const size_t per_state =
+      p->rowpath ? (size_t)nz * p->geo.rstride * sizeof(double2)
+                 : (size_t)p->geo.nxa * p->geo.ps * sizeof(double2);
   // Chunking of the FFT sweeps.  When the plane buffer can hold every state
   // (default budget 16 GiB), the forward sweep's planes are kept and the
   // adjoint sweep starts from them instead of transforming Q again.
@@ -325,23 +370,27 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   else if (chunk > spin_states) chunk = spin_states;
   if (chunk > 65535) chunk = 65535;
   p->chunk = (int)chunk;
-  // JRX_FFT_GENERIC=1 selects the runtime-radix FFT kernels even where the
-  // register-resident 64-point ones apply (used to cross-check them).
-  const char* gen = getenv("JRX_FFT_GENERIC");
-  JRX_REQUIRE(d->fft_variant == 0 || d->fft_variant == 1,
-              "fft_variant must be 0 (automatic) or 1 (runtime radix)");
   JRX_REQUIRE(d->xline_blocks >= 0, "xline_blocks must be >= 0");
   JRX_REQUIRE(d->qr_passes == 0 || d->qr_passes == 2,
               "qr_passes must be 0 (adaptive) or 2");
   p->qr_adaptive = d->qr_passes == 0;
-  p->force_generic = (gen && gen[0] == '1') || d->fft_variant == 1;
   p->xline_blocks = d->xline_blocks > 0 ? d->xline_blocks : 0;
-  p->groups = xline_groups(p->geo,
-                           p->keep_planes ? (int)spin_states : p->chunk,
-                           p->xline_blocks);
+  const int sweep_states = p->keep_planes ? (int)spin_states : p->chunk;
+  p->groups = p->rowpath ? zplane_groups(p->geo, sweep_states)
+                         : xline_groups(p->geo, sweep_states, p->xline_blocks);
+  p->row_chunks = p->rowpath ? row_chunks(p->geo) : 0;
+  // z-major cell of the row path: box axes (z, y, x)
+  p->cpz = p->cp;
+  p->cpz.nx = nz;
+  p->cpz.nz = nx;
+  for (int i = 0; i < 3; ++i) {
+    p->cpz.B[i] = p->B[6 + i];
+    p->cpz.B[6 + i] = p->B[i];
+  }
   p->gsw = gram_split(p->P, ng);
   p->gsq = p->gsw;
-  const int64_t kin_gather = p->S * p->geo.nxa;
+  const int64_t kin_gather =
+      p->S * (p->rowpath ? p->row_chunks : p->geo.nxa);
   const int64_t kin_plain = p->S * ((ng + kRedBlock - 1) / kRedBlock);
   p->kin_parts_max = (int)(kin_gather > kin_plain ? kin_gather : kin_plain);
   p->rec_parts = (int)((p->N + kRedBlock - 1) / kRedBlock);
@@ -373,8 +422,9 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_ee = take((size_t)p->rec_parts * sizeof(double));
   p->o_exc = take((size_t)p->rec_parts * sizeof(double));
   p->o_stats = take((size_t)(2 * kStatBlocks + 8) * sizeof(double));
+  if (xc_is_gga(p->xc)) p->o_gb = take(3 * (size_t)p->N * sizeof(double2));
   p->o_qrflag = take((size_t)p->P * sizeof(int));
-  p->o_bepart = take((size_t)p->S * p->geo.nxa * sizeof(double));
+  p->o_bepart = take((size_t)kin_gather * sizeof(double));
   p->o_pot = take((size_t)p->S * sizeof(double));
   p->total = off;
   return JRX_OK;
@@ -453,6 +503,8 @@ int jrx_plan_get_info(const jrx_plan* plan, jrx_plan_info* info) {
   info->chunk_states = plan->chunk;
   info->keep_planes = plan->keep_planes ? 1 : 0;
   info->workspace_bytes = plan->total;
+  info->row_path = plan->rowpath ? 1 : 0;
+  info->rows = plan->geo.nrows;
   return JRX_OK;
 }
 
@@ -1054,7 +1106,9 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   // --- density -------------------------------------------------------------
   const int64_t SS = (int64_t)p->nk * nb;  // states per spin
   const bool reg64 = use_reg64(p->geo) && !p->force_generic;
-  const int64_t plane_elems = (int64_t)p->geo.nxa * p->geo.ps;
+  const int64_t plane_elems =
+      p->rowpath ? (int64_t)p->nz * p->geo.rstride
+                 : (int64_t)p->geo.nxa * p->geo.ps;
   for (int sp = 0; sp < p->ns; ++sp) {
     JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
     const int64_t step = p->keep_planes ? SS : p->chunk;
@@ -1062,7 +1116,14 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
       const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
       const int64_t st0 = sp * SS + s0;
       double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
-      if (reg64) {
+      if (p->rowpath) {
+        JRX_T(JRX_K_PLANE_IFFT,
+              launch_row_ifft(q, Tc, st0, nst, p->geo, p->Lx.tw, s));
+        JRX_T(JRX_K_XLINE_DENSITY,
+              launch_zplane_density(Tc, rpart, zplane_groups(p->geo, nst),
+                                    p->d_occ, st0, nst, 1.0 / p->vol,
+                                    p->geo, p->Lx.tw, s));
+      } else if (reg64) {
         JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst, p->geo,
                                                     p->Lx.tw, s));
         JRX_T(JRX_K_XLINE_DENSITY,
@@ -1082,15 +1143,29 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
           launch_reduce_groups(rpart, rho + sp * p->N, p->groups, p->N, s));
   }
   // --- reciprocal energies and potential ----------------------------------
+  // (on the row path rho and v are z-major: the box axes are (z, y, x))
+  const CellParams& cpb = p->rowpath ? p->cpz : p->cp;
+  const FftLen& L0 = p->rowpath ? p->Lz : p->Lx;
+  const FftLen& L2 = p->rowpath ? p->Lx : p->Lz;
   JRX_T(JRX_K_POTENTIAL, launch_rho_to_complex(rho, box, p->ns, p->N, s));
-  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, p->nx, p->ny, p->nz, false, 1.0, p->Lx, p->Ly,
-                p->Lz, s));
+  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, false,
+                               1.0, L0, p->Ly, L2, s));
   int nrec = 0, nexc = 0, nkin = 0;
-  JRX_T(JRX_K_POTENTIAL, launch_potential_rec(box, box, eh, ee, p->cp, &nrec, s));
-  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, p->nx, p->ny, p->nz, true, 1.0 / (double)p->N,
-                p->Lx, p->Ly, p->Lz, s));
-  JRX_T(JRX_K_POTENTIAL, launch_potential_real(box, rho, v, exc, p->ns, p->N, p->vol, p->cx,
-                                &nexc, s));
+  double2* gb = p->o_gb ? cptr(p->o_gb) : nullptr;
+  if (gb)  // density gradients of the GGA terms, from fftn(rho)
+    JRX_T(JRX_K_POTENTIAL,
+          xc_stage_gradients(box, gb, cpb, L0, p->Ly, L2, s));
+  JRX_T(JRX_K_POTENTIAL, launch_potential_rec(box, box, eh, ee, cpb, &nrec, s));
+  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, true,
+                               1.0 / (double)p->N, L0, p->Ly, L2, s));
+  if (p->xc == JRX_XC_LDA_X)
+    JRX_T(JRX_K_POTENTIAL, launch_potential_real(box, rho, v, exc, p->ns,
+                                                 p->N, p->vol, p->cx, &nexc,
+                                                 s));
+  else
+    JRX_T(JRX_K_POTENTIAL,
+          xc_stage_potential(box, rho, v, gb, exc, &nexc, cpb, p->xc, L0,
+                             p->Ly, L2, s));
   if (dw_re) {
     // --- adjoint sweep: qbar = f (2/vol FFT(v psi) + |G+k|^2 q) -------------
     GatherParams gp;
@@ -1101,13 +1176,25 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
     gp.nb = nb;
     gp.two_over_vol = 2.0 / p->vol;
     gp.pot_partial = p->band_e ? dptr(p->o_bepart) : nullptr;
+    gp.g_m = p->d_g_m;
     const int64_t step = p->keep_planes ? SS : p->chunk;
     for (int sp = 0; sp < p->ns; ++sp) {
       for (int64_t s0 = 0; s0 < SS; s0 += step) {
         const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
         const int64_t st0 = sp * SS + s0;
         double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
-        if (reg64) {
+        if (p->rowpath) {
+          if (!p->keep_planes)
+            JRX_T(JRX_K_PLANE_IFFT,
+                  launch_row_ifft(q, Tc, st0, nst, p->geo, p->Lx.tw, s));
+          JRX_T(JRX_K_XLINE_VPSI,
+                launch_zplane_vpsi(Tc, v + sp * p->N,
+                                   zplane_groups(p->geo, nst), nst, p->geo,
+                                   p->Lx.tw, s));
+          JRX_T(JRX_K_PLANE_FFT_GATHER,
+                launch_row_fft_gather(Tc, q, qbar, kin, st0, nst, p->geo,
+                                      p->Lx.tw, gp, s));
+        } else if (reg64) {
           if (!p->keep_planes)
             JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst,
                                                         p->geo, p->Lx.tw, s));
@@ -1131,9 +1218,10 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
         }
       }
     }
-    JRX_T(JRX_K_OTHER, launch_state_sums(kin, p->geo.nxa, S, tb, s));
+    const int nper = p->rowpath ? p->row_chunks : p->geo.nxa;
+    JRX_T(JRX_K_OTHER, launch_state_sums(kin, nper, S, tb, s));
     if (p->band_e)
-      JRX_T(JRX_K_OTHER, launch_state_sums(dptr(p->o_bepart), p->geo.nxa, S,
+      JRX_T(JRX_K_OTHER, launch_state_sums(dptr(p->o_bepart), nper, S,
                                            dptr(p->o_pot), s));
     // --- QR adjoint ----------------------------------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
@@ -1147,10 +1235,14 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   }
   JRX_T(JRX_K_OTHER, launch_finalize_energies(tb, p->d_occ, S, eh, ee, nrec,
                                               exc, nexc, energies, s));
-  if (rho_out)
+  if (rho_out && p->rowpath) {
+    if ((rc = launch_zmajor_to_xmajor(rho, rho_out, p->ns, p->nx, s)))
+      return rc;
+  } else if (rho_out) {
     JRX_CHECK_HIP(hipMemcpyAsync(rho_out, rho,
                                  (size_t)p->ns * p->N * sizeof(double),
                                  hipMemcpyDeviceToDevice, s));
+  }
   if (q_out)
     JRX_CHECK_HIP(hipMemcpyAsync(q_out, q, (size_t)S * ng * sizeof(double2),
                                  hipMemcpyDeviceToDevice, s));
@@ -1186,4 +1278,3 @@ int jrx_energy_grad_static(jrx_plan* p, const double* w_re,
 }
 
 }  // extern "C"
-

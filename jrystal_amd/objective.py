@@ -32,6 +32,23 @@ def _require_cuda():
                        "(no CPU fallback)")
 
 
+# JRX_XC_* flags of include/jrx.h for the reference's xc term names
+XC_TERMS = {"lda_x": 1, "lda_c_pw": 2, "gga_x_pbe": 4, "gga_c_pbe": 8}
+
+
+def xc_terms(xc: str) -> int:
+  """Flags of a '+'-joined xc string (xc.py:222-253 splits it the same
+  way); terms outside XC_TERMS raise NotImplementedError."""
+  flags = 0
+  for t in xc.split("+"):
+    t = t.strip()
+    if t not in XC_TERMS:
+      raise NotImplementedError(
+        f"xc term {t!r}: this path has {sorted(XC_TERMS)}")
+    flags |= XC_TERMS[t]
+  return flags
+
+
 class EnergyPlan:
   """Plan for one (crystal, mask, k-points, occupation) problem.
 
@@ -52,8 +69,7 @@ class EnergyPlan:
                device=None, fft_variant: int = 0, xline_blocks: int = 0,
                plane_pad: int = -1, qr_passes: int = 0):
     _require_cuda()
-    if xc != "lda_x":
-      raise NotImplementedError(f"xc {xc!r}: only lda_x is on this path")
+    xc_flags = xc_terms(xc)
     self.device = torch.device(device if device is not None else "cuda")
     self._lib = _lib.lib()
     mask = np.ascontiguousarray(np.asarray(mask, dtype=bool))
@@ -82,7 +98,8 @@ class EnergyPlan:
     d.positions = pos.ctypes.data
     d.charges = chg.ctypes.data
     d.occupation = occ.ctypes.data
-    d.xc = 0
+    d.xc = xc_flags
+    self.xc = xc
     d.chunk_states = int(chunk_states)
     d.fft_variant = int(fft_variant)
     d.xline_blocks = int(xline_blocks)
@@ -103,6 +120,8 @@ class EnergyPlan:
     self.active_planes = info.active_planes
     self.chunk_states = info.chunk_states
     self.keep_planes = bool(info.keep_planes)
+    self.row_path = bool(info.row_path)
+    self.num_rows = info.rows
     self.ns, self.nk, self.nb = occ.shape
     self.occupation = occ.copy()
     self.grid_sizes = mask.shape

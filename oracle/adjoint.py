@@ -72,7 +72,10 @@ def potential_real(rho, prob: Problem):
   ve_rec = s_conj / (g2 + 1e-10)
   ve_rec[0, 0, 0] = 0.0
   v_e = -4 * np.pi * np.real(sfft.ifftn(ve_rec, workers=_pw._WORKERS))
-  if rho.shape[0] == 1:
+  if prob.xc != "lda_x":
+    from . import xc as _xc
+    v_xc = _xc.xc_potential(rho, g, vol, prob.xc)
+  elif rho.shape[0] == 1:
     v_xc = (vol / n) * (4.0 / 3.0) * _en.lda_x_eps(rho)
   else:
     # E = vol/N sum eps(r) rho_tot, eps = 0.5 (e(2 rho_u) + e(2 rho_d))

@@ -95,9 +95,9 @@ def lda_x_eps(rho):
 def xc_density(density_grid, xc_type="lda_x"):
   """xc.py:222-253 for LDA exchange.  Spin-polarised input uses the
   reference's exchange scaling 0.5 * (eps(2 rho_up) + eps(2 rho_dn))
-  (xc.py:56-59)."""
+  (xc.py:56-59).  Other terms: oracle/xc.py (via xc_energy)."""
   if xc_type != "lda_x":
-    raise NotImplementedError(f"xc {xc_type!r}: only lda_x is on this path")
+    raise NotImplementedError(f"xc {xc_type!r}: see oracle.xc.xc_energy")
   if density_grid.shape[0] == 2:
     return 0.5 * (lda_x_eps(2 * density_grid[0]) +
                   lda_x_eps(2 * density_grid[1]))
@@ -109,6 +109,9 @@ def xc_energy(density_grid, g_vector_grid, vol, xc_type="lda_x",
   """energy.py:185-211: sum_r eps(rho) * rho_s summed over spin, * vol/N."""
   assert density_grid.ndim == 4
   n = np.prod(density_grid.shape[-3:])
+  if xc_type != "lda_x":
+    from . import xc as _xc
+    return _xc.xc_energy(density_grid, g_vector_grid, vol, xc_type)
   exc = xc_density(density_grid, xc_type)
   return safe_real(np.sum(exc[None] * density_grid)) * vol / n
 

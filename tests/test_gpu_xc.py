@@ -1,0 +1,89 @@
+"""Exchange-correlation terms beyond LDA exchange on the GPU (PW92
+correlation, PBE exchange and correlation) against the oracle (oracle/xc.py,
+pinned by published limits and central differences): energies, density and
+gradient of the fused evaluation on the plane path (Si2, 32^3) and on the
+row path (Si8 cube, 64^3, z-major potential); the driver with the
+reference's default xc = gga_x_pbe+gga_c_pbe; spin-polarised problems with
+these terms are rejected.  Tolerances: total energy 1e-8 Ha, density 1e-10,
+gradient 1e-7 relative."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from helpers import oracle_problem, rel_max, seeded_params
+from oracle import adjoint as OA
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+NAMES = ("kinetic", "hartree", "external", "xc", "total")
+
+
+def _case(wl, xc, seed):
+  from jrystal_amd.objective import EnergyPlan
+  plan = EnergyPlan.from_workload(wl) if xc == "lda_x" else EnergyPlan(
+    wl.crystal.positions, wl.crystal.charges, wl.crystal.cell_vectors,
+    wl.mask, wl.kpts, wl.occupation, xc=xc)
+  p = seeded_params(wl.shape, seed)
+  out = plan.evaluate(torch.tensor(p["w_re"], device="cuda"),
+                      torch.tensor(p["w_im"], device="cuda"), grad=True,
+                      density=True)
+  res = {k: v.cpu().numpy() for k, v in out.items()}
+  prob = oracle_problem(wl.crystal, wl.grid_sizes, wl.k_grid, wl.num_bands,
+                        wl.cutoff_energy)
+  prob = dataclasses.replace(prob, xc=xc)
+  parts, grad, rho = OA.energy_and_grad(p, prob)
+  print(f"{wl.name} {xc}: E_xc {res['energies'][3]:.12f} "
+        f"oracle {parts['xc']:.12f}", flush=True)
+  for i, k in enumerate(NAMES):
+    assert abs(res["energies"][i] - parts[k]) <= 1e-8, (k, res["energies"][i],
+                                                         parts[k])
+  assert np.max(np.abs(res["density"] - rho)) <= 1e-10
+  assert rel_max(res["dw_re"], grad["w_re"]) <= 1e-7
+  assert rel_max(res["dw_im"], grad["w_im"]) <= 1e-7
+  return plan, res
+
+
+@pytest.mark.parametrize("xc", ["lda_x+lda_c_pw", "gga_x_pbe", "gga_c_pbe",
+                                "gga_x_pbe+gga_c_pbe"])
+def test_xc_terms_si2_32(xc):
+  from jrystal_amd import workloads as W
+  _case(W.si2_fcc(4.0), xc, seed=3)
+
+
+def test_gga_row_path_si8_64():
+  from jrystal_amd import workloads as W
+  plan, _ = _case(W.si8_cube(4.0, k=1), "gga_x_pbe+gga_c_pbe", seed=5)
+  assert plan.row_path
+
+
+def test_gga_spin_polarised_rejected():
+  from jrystal_amd import occupation, workloads as W
+  from jrystal_amd.objective import EnergyPlan
+  wl = W.si2_fcc(4.0)
+  occ = occupation.uniform(wl.kpts.shape[0], wl.crystal.num_electron, 0,
+                           wl.num_bands, spin_restricted=False)
+  with pytest.raises(NotImplementedError, match="unpolarised"):
+    EnergyPlan(wl.crystal.positions, wl.crystal.charges,
+               wl.crystal.cell_vectors, wl.mask, wl.kpts, occ,
+               xc="gga_x_pbe")
+  with pytest.raises(NotImplementedError, match="mgga"):
+    EnergyPlan(wl.crystal.positions, wl.crystal.charges,
+               wl.crystal.cell_vectors, wl.mask, wl.kpts, wl.occupation,
+               xc="mgga_x_scan")
+
+
+def test_cli_with_reference_default_xc(tmp_path):
+  from jrystal_amd import config
+  from jrystal_amd.__main__ import main
+  from jrystal_amd.calc.energy_all_electrons import calc_with_output
+  p = tmp_path / "c.yaml"
+  p.write_text("crystal: si\ngrid_sizes: 16\nk_grid_sizes: 1\n"
+               "cutoff_energy: 10\nempty_bands: 2\nepoch: 40\n"
+               "verbose: false\nseed: 1\nxc: gga_x_pbe+gga_c_pbe\n")
+  assert main(["-m", "energy", "-c", str(p)]) == 0
+  out = calc_with_output(config.get_config(str(p)))
+  h = out.total_energy_history
+  assert len(h) == 40 and h[-1] < h[0]
+  assert np.isfinite(out.energies["xc"]) and out.energies["xc"] < 0
