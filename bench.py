@@ -52,13 +52,14 @@ def algorithmic_bytes(cls, plan, chunks):
   Q = S * ng * 16                 # orthonormal coefficients
   W = 2 * S * ng * 8              # planar parameters (or gradient)
   rho = plan.ns * N * 8
-  if cls == "plane_ifft":         # reads Q, writes T, once per sweep
+  # (T: the plane buffer, or on the row path the row buffer R)
+  if cls in ("plane_ifft", "row_ifft"):   # reads Q, writes T, per sweep
     return (1 if plan.keep_planes else 2) * (Q + T)
-  if cls == "xline_density":      # read T; rho read+write once per chunk
+  if cls in ("xline_density", "zplane_density"):  # read T; rho per chunk
     return T + chunks * 2 * rho
-  if cls == "xline_vpsi":         # read T, write T, read v once per chunk
+  if cls in ("xline_vpsi", "zplane_vpsi"):  # read T, write T, read v
     return 2 * T + chunks * rho
-  if cls == "plane_fft_gather":   # read T, read Q, write Qbar
+  if cls in ("plane_fft_gather", "row_fft_gather"):  # T, Q in; Qbar out
     return T + 2 * Q
   if cls == "qr_gram":            # two Gram passes over W
     return 2 * W
@@ -259,7 +260,16 @@ TRAFFIC_PROFILE = os.path.join(
 KERNEL_NAMES = {"xline_vpsi": "jrx::xline_vpsi64_kernel",
                 "plane_ifft": "jrx::plane_ifft64_kernel",
                 "xline_density": "jrx::xline_density64_kernel",
-                "plane_fft_gather": "jrx::plane_fft_gather64_kernel"}
+                "plane_fft_gather": "jrx::plane_fft_gather64_kernel",
+                "zplane_vpsi": "jrx::zplane_vpsi_kernel<64, 1>",
+                "row_ifft": "jrx::row_ifft_kernel<64>",
+                "zplane_density": "jrx::zplane_density_kernel<64, 1>",
+                "row_fft_gather": "jrx::row_fft_gather_kernel<64, false>"}
+# On the row-buffer path the four FFT timing classes hold these kernels.
+ROW_PATH_CLASSES = {"plane_ifft": "row_ifft",
+                    "xline_density": "zplane_density",
+                    "xline_vpsi": "zplane_vpsi",
+                    "plane_fft_gather": "row_fft_gather"}
 
 
 def measured_traffic(cls):
@@ -374,6 +384,8 @@ def main():
     plan.evaluate(wr, wi, grad=True, out=out)
   timing = plan.read_timing()
   plan.set_timing(False)
+  if plan.row_path:
+    timing = {ROW_PATH_CLASSES.get(k, k): v for k, v in timing.items()}
   chunks = plan.ns * max(1, -(-plan.nk * plan.nb // plan.chunk_states))
   dom = max(timing, key=lambda k: timing[k][0])
   ms, launches = timing[dom]

@@ -312,6 +312,25 @@ __device__ __forceinline__ void fft64_8x8(double2 (&v)[8], double2* base,
   Dft<8, INV>::run(v, nullptr, 0);
 }
 
+// fft64_8x8 with the lane's twiddles w[a] = w64^{j a} (a = 1..7, forward
+// sign) held in registers, and exchange slot s at addr(s) (any one-to-one
+// map of the 64 slots, e.g. a bank swizzle).
+template <bool INV, class Addr>
+__device__ __forceinline__ void fft64_8x8_rw(double2 (&v)[8], Addr addr,
+                                             int j, const double2 (&w)[8]) {
+  Dft<8, INV>::run(v, nullptr, 0);
+#pragma unroll
+  for (int a = 1; a < 8; ++a) v[a] = cmul(v[a], INV ? cconj(w[a]) : w[a]);
+  wave_sync();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) *addr(8 * a + ((j + a) & 7)) = v[a];
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = *addr(8 * j + ((i + j) & 7));
+  wave_sync();
+  Dft<8, INV>::run(v, nullptr, 0);
+}
+
 // 16-point DFT in registers: radix-2 split into two 8-point DFTs, with the
 // roots w16^t = tw[8t] taken from the length-128 table.
 template <bool INV>
@@ -341,26 +360,26 @@ __device__ __forceinline__ void dft16(double2 (&u)[16],
 // The exchange goes through 64 LDS slots base[s * es] in two rounds
 // (a < 8, then a >= 8), so a line needs the same 64 slots as a 64-point
 // one.  All lanes of a line are in one wave.
-template <bool INV>
-__device__ __forceinline__ void fft128_16x8(double2 (&v)[16], double2* base,
-                                            int es, int j,
-                                            const double2* __restrict__ tw) {
+template <bool INV, class Addr>
+__device__ __forceinline__ void fft128_16x8_a(double2 (&v)[16], Addr addr,
+                                              int j,
+                                              const double2* __restrict__ tw) {
   dft16<INV>(v, tw);
 #pragma unroll
   for (int a = 1; a < 16; ++a) v[a] = cmul(v[a], twid<INV>(tw, j * a));
   double2 z0[8], z1[8];
   wave_sync();
 #pragma unroll
-  for (int a = 0; a < 8; ++a) base[(8 * a + ((j + a) & 7)) * es] = v[a];
+  for (int a = 0; a < 8; ++a) *addr(8 * a + ((j + a) & 7)) = v[a];
   wave_sync();
 #pragma unroll
-  for (int i = 0; i < 8; ++i) z0[i] = base[(8 * j + ((i + j) & 7)) * es];
+  for (int i = 0; i < 8; ++i) z0[i] = *addr(8 * j + ((i + j) & 7));
   wave_sync();
 #pragma unroll
-  for (int a = 0; a < 8; ++a) base[(8 * a + ((j + a) & 7)) * es] = v[a + 8];
+  for (int a = 0; a < 8; ++a) *addr(8 * a + ((j + a) & 7)) = v[a + 8];
   wave_sync();
 #pragma unroll
-  for (int i = 0; i < 8; ++i) z1[i] = base[(8 * j + ((i + j) & 7)) * es];
+  for (int i = 0; i < 8; ++i) z1[i] = *addr(8 * j + ((i + j) & 7));
   wave_sync();
   Dft<8, INV>::run(z0, nullptr, 0);
   Dft<8, INV>::run(z1, nullptr, 0);
@@ -369,6 +388,12 @@ __device__ __forceinline__ void fft128_16x8(double2 (&v)[16], double2* base,
     v[2 * b] = z0[b];
     v[2 * b + 1] = z1[b];
   }
+}
+template <bool INV>
+__device__ __forceinline__ void fft128_16x8(double2 (&v)[16], double2* base,
+                                            int es, int j,
+                                            const double2* __restrict__ tw) {
+  fft128_16x8_a<INV>(v, [=](int s) { return base + s * es; }, j, tw);
 }
 
 // Register transform of one line of N points on eight lanes (N/8 values

@@ -26,7 +26,8 @@ struct BoxGeom {
   // contiguous in g.
   int nrows;
   int rstride;          // elements per z-plane of one state in the row
-                        // buffer R[state][z][row] (nrows rounded up to 8)
+                        // buffer R[state][z][row] (nrows + 1 rounded up to
+                        // 8: element nrows is a pad that nothing reads)
   const int* row_g0;    // [nrows+1] first mask point of each row
   const int* g_row;     // [ng]      row of each mask point
   const int* rowidx;    // [nxa][ny] row of (active plane, y), or -1

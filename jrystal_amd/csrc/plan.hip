@@ -285,7 +285,9 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   if ((rc = upload(p, xmap.data(), xmap.size(), &tmp))) return rc;
   p->geo.xmap = tmp;
   p->geo.nrows = nrows;
-  p->geo.rstride = (nrows + 7) / 8 * 8;
+  // (at least one pad element per z-plane slice: the z-plane passes store
+  // their off-mask lanes there)
+  p->geo.rstride = (nrows + 1 + 7) / 8 * 8;
   if ((rc = upload(p, row_g0.data(), row_g0.size(), &tmp))) return rc;
   p->geo.row_g0 = tmp;
   if ((rc = upload(p, g_row.data(), g_row.size(), &tmp))) return rc;
@@ -325,9 +327,9 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   JRX_CHECK_HIP(hipDeviceGetAttribute(
       &max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
   const int nb = d->nb;
-  // (the row path's largest block holds a 128 x 66 z-plane)
+  // (the row path's largest block holds an nz x 72 z-plane and twiddles)
   const size_t lds_plane =
-      (p->rowpath ? (size_t)nz * 66 : (size_t)ny * nz) * sizeof(double2);
+      (p->rowpath ? (size_t)nz * 73 : (size_t)ny * nz) * sizeof(double2);
   const size_t nbp = 16 * (size_t)((nb + 15) / 16);
   const size_t lds_qr = 2 * nbp * nbp * sizeof(double2);
   const size_t lds_gram = 2 * nbp * nbp * sizeof(double2);  // pre + sums

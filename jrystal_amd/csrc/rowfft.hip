@@ -20,8 +20,17 @@
 
 namespace jrx {
 
+// LDS layouts, chosen by measuring the bank-conflict counter on gfx950 for
+// the exact exchange patterns of these kernels (conflict cycles per 16-byte
+// access of one wave): line exchanges in rows of stride 72 cost 0 (stride
+// 66: 9 on reads); column exchanges through a stride-72 plane cost 56 on
+// writes unless the column position is XOR-swizzled by the row
+// (ZPlane::at: 0 on writes, 3 on reads); transposed tiles of stride
+// RPB + 1 cost 0 on writes and 4 on reads.
 // LDS stride (elements) of one row of a z-plane: 64 exchange slots + pad.
-constexpr int kZRow = 66;
+constexpr int kZRow = 72;
+// row-pass staging rows (N + kRowPad) and transposed tiles (RPB + 1)
+constexpr int kRowPad = 8;
 
 template <int N>
 struct RowCfg;  // rows per block of the row passes (8 lanes per row)
@@ -34,21 +43,43 @@ struct RowCfg<128> {
   static constexpr int RPB = 32;
 };
 
+// Launch shape of the 64-point z-plane passes (compile-time tuning knobs):
+// threads per block, the waves per SIMD the register budget is set for (0:
+// the compiler's choice), branch-free row gathers / scatters, and register
+// prefetch of the next state's column inputs.  Measured at 64^3 (64 k, 32
+// bands): 256 threads with prefetch and branches 2.07 ms (v psi pass);
+// 512 threads at 4 waves per SIMD 2.61 ms; branch-free forms 3.05-3.23 ms.
+#ifndef JRX_ZBLOCK64
+#define JRX_ZBLOCK64 256
+#endif
+#ifndef JRX_ZWPE64
+#define JRX_ZWPE64 0
+#endif
+#ifndef JRX_ZBRANCHFREE
+#define JRX_ZBRANCHFREE 0
+#endif
+#ifndef JRX_ZPREFETCH64
+#define JRX_ZPREFETCH64 1
+#endif
+
 template <int N>
-struct ZCfg;  // threads of the z-plane passes (8 lanes per line)
+struct ZCfg;  // threads of the z-plane passes (8 lanes per line) and the
+              // waves per SIMD the register budget is set for
 template <>
 struct ZCfg<64> {
-  static constexpr int BLOCK = 256;
+  static constexpr int BLOCK = JRX_ZBLOCK64, WPE = JRX_ZWPE64;
+  static constexpr bool PF = JRX_ZPREFETCH64 != 0;
 };
 template <>
 struct ZCfg<128> {
-  static constexpr int BLOCK = 512;
+  static constexpr int BLOCK = 512, WPE = 2;  // LDS: one block per CU
+  static constexpr bool PF = false;
 };
 
 template <int N>
 constexpr int row_lds_elems() {
   constexpr int RPB = RowCfg<N>::RPB;
-  constexpr int a = RPB * (N + 2), b = N * (RPB + 1);
+  constexpr int a = RPB * (N + kRowPad), b = N * (RPB + 1);
   return (a > b ? a : b) + N;  // + twiddles
 }
 
@@ -61,7 +92,7 @@ __global__ __launch_bounds__(RowCfg<N>::RPB * 8) void row_ifft_kernel(
     const double2* __restrict__ q, double2* __restrict__ R, int64_t state0,
     BoxGeom geo, const double2* __restrict__ tw) {
   constexpr int RPB = RowCfg<N>::RPB, NT = RPB * 8, V = N / 8;
-  constexpr int RS = N + 2, TS = RPB + 1;
+  constexpr int RS = N + kRowPad, TS = RPB + 1;
   extern __shared__ double2 sm[];
   double2* stw = sm + (row_lds_elems<N>() - N);
   const int tid = threadIdx.x, j = tid & 7, rl = tid >> 3;
@@ -180,10 +211,9 @@ template <int N, int CP>
 struct ZPlane {
   static constexpr int BLOCK = ZCfg<N>::BLOCK, V = N / 8, LPB = BLOCK / 8;
   static constexpr int YP = N / LPB;  // line passes
-  // The next state's column inputs are prefetched into registers while the
-  // current state is transformed (64-point lines only: at 128 points the
-  // registers are taken by the transforms and the accumulators).
-  static constexpr bool PF = N == 64;
+  // Prefetch of the next state's column inputs into registers while the
+  // current state is transformed.
+  static constexpr bool PF = ZCfg<N>::PF;
   static constexpr int PV = PF ? V : 1;
   // The density pass at 128 points runs the x transforms of each z-plane in
   // two blocks (y halves; both redo the column transforms), which halves
@@ -193,29 +223,85 @@ struct ZPlane {
   static constexpr int YPD = N / YS / LPB;  // line passes of a density block
   static constexpr bool VREG = N == 64;
 
+  // Per-lane index tables held as pairs of 16-bit signed values per
+  // register (row indices < 4096 and x-plane indices < 64 fit), -1 = off
+  // the mask.
+  struct Packed {
+    unsigned w[V / 2];
+    __device__ int operator[](int m) const {
+      return (int)(short)(w[m >> 1] >> (16 * (m & 1)));
+    }
+    __device__ void set(int m, int v) {
+      const unsigned h = (unsigned)(v & 0xffff) << (16 * (m & 1));
+      w[m >> 1] = (m & 1) ? ((w[m >> 1] & 0xffffu) | h) : h;
+    }
+  };
   // Row index of (column ln + p*LPB, y = j + 8m), or -1.
-  __device__ static void row_table(int (&ri)[CP][V], const BoxGeom& geo,
+  __device__ static void row_table(Packed (&ri)[CP], const BoxGeom& geo,
                                    int ln, int j) {
 #pragma unroll
     for (int p = 0; p < CP; ++p) {
       const int xa = ln + p * LPB;
 #pragma unroll
       for (int m = 0; m < V; ++m)
-        ri[p][m] = xa < geo.nxa ? geo.rowidx[xa * N + j + 8 * m] : -1;
+        ri[p].set(m, xa < geo.nxa ? geo.rowidx[xa * N + j + 8 * m] : -1);
     }
   }
-  __device__ static void load(double2 (&pre)[CP][V], const int (&ri)[CP][V],
+  __device__ static void x_table(Packed& xo, const BoxGeom& geo, int j) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) xo.set(m, geo.xmap[j + 8 * m]);
+  }
+  // Loads without branches: off-mask lanes read row 0 and drop the value.
+  __device__ static void load(double2 (&pre)[CP][V], const Packed (&ri)[CP],
                               const double2* src) {
 #pragma unroll
     for (int p = 0; p < CP; ++p)
 #pragma unroll
-      for (int m = 0; m < V; ++m)
-        pre[p][m] = ri[p][m] >= 0 ? nt_load(src + ri[p][m]) : czero();
+      for (int m = 0; m < V; ++m) {
+        const int r = ri[p][m];
+        if (JRX_ZBRANCHFREE) {
+          const double2 a = nt_load(src + (unsigned)(r < 0 ? 0 : r));
+          pre[p][m] = r < 0 ? czero() : a;
+        } else {
+          pre[p][m] = r >= 0 ? nt_load(src + r) : czero();
+        }
+      }
+  }
+  // Element (y, xa) of P sits at y * kZRow + (xa ^ (y & 7)) (see kZRow).
+  __device__ static int at(int y, int xa) {
+    return y * kZRow + (xa ^ (y & 7));
+  }
+  // Exchange slots of the transform of column xa, and of line y.
+  struct ColAddr {
+    double2* sm;
+    int xa;
+    __device__ double2* operator()(int s) const { return sm + at(s, xa); }
+  };
+  struct LineAddr {
+    double2* row;
+    __device__ double2* operator()(int s) const { return row + s; }
+  };
+  // Lane twiddles of the 64-point transform kept in registers (the
+  // 128-point one reads its table: its registers are taken).
+  struct Tw {
+    double2 w[8];
+    __device__ void init(const double2* stw, int j) {
+      if constexpr (N == 64) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) w[a] = stw[(j * a) & 63];
+      }
+    }
+  };
+  template <bool INV, class Addr>
+  __device__ static void fft(double2 (&v)[V], Addr addr, int j,
+                             const double2* stw, const Tw& tw) {
+    if constexpr (N == 64) fft64_8x8_rw<INV>(v, addr, j, tw.w);
+    else fft128_16x8_a<INV>(v, addr, j, stw);
   }
   // Inverse y transform of the active columns (inputs in pre) into P.
   __device__ static void y_inverse(double2* sm, const double2 (&pre)[CP][V],
                                    const BoxGeom& geo, int ln, int j,
-                                   const double2* stw) {
+                                   const double2* stw, const Tw& tw) {
 #pragma unroll
     for (int p = 0; p < CP; ++p) {
       const int xa = ln + p * LPB;
@@ -223,22 +309,32 @@ struct ZPlane {
         double2 v[V];
 #pragma unroll
         for (int m = 0; m < V; ++m) v[m] = pre[p][m];
-        RegFft<N>::template run<true>(v, sm + xa, kZRow, j, stw);
+        fft<true>(v, ColAddr{sm, xa}, j, stw, tw);
 #pragma unroll
-        for (int t = 0; t < V; ++t) sm[(j + 8 * t) * kZRow + xa] = v[t];
+        for (int t = 0; t < V; ++t) sm[at(j + 8 * t, xa)] = v[t];
       }
     }
   }
   // Line y of P into registers: v[m] = P[y][xmap[j + 8m]] (0 off the mask).
-  __device__ static void line_load(double2 (&v)[V], const double2* row,
-                                   const int (&xo)[V]) {
+  __device__ static void line_load(double2 (&v)[V], const double2* sm, int y,
+                                   const Packed& xo) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) v[m] = xo[m] >= 0 ? row[xo[m]] : czero();
+    for (int m = 0; m < V; ++m) {
+      const int x = xo[m];
+      if (JRX_ZBRANCHFREE) {
+        const double2 a = sm[at(y, x < 0 ? 0 : x)];
+        v[m] = x < 0 ? czero() : a;
+      } else {
+        v[m] = x >= 0 ? sm[at(y, x)] : czero();
+      }
+    }
   }
 };
 
 template <int N, int CP>
-__global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_density_kernel(
+__global__ __launch_bounds__(ZCfg<N>::BLOCK,
+                            ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
+zplane_density_kernel(
     const double2* __restrict__ R, double* __restrict__ rho_part,
     const double* __restrict__ occ, int64_t state0, int nstates,
     double inv_vol, BoxGeom geo, const double2* __restrict__ tw) {
@@ -255,11 +351,11 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_density_kernel(
   for (int i = tid; i < N; i += BLOCK) stw[i] = tw[i];
   __syncthreads();
   if (cb >= ce) return;  // uniform over the block
-  int ri[CP][V];
+  typename Z::Packed ri[CP], xo;
   Z::row_table(ri, geo, ln, j);
-  int xo[V];
-#pragma unroll
-  for (int m = 0; m < V; ++m) xo[m] = geo.xmap[j + 8 * m];
+  Z::x_table(xo, geo, j);
+  typename Z::Tw rtw;
+  rtw.init(stw, j);
   double acc[YP][V];
 #pragma unroll
   for (int p = 0; p < YP; ++p)
@@ -281,16 +377,17 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_density_kernel(
       } else {
         Z::load(cur, ri, src0 + (int64_t)c * sstride);
       }
-      Z::y_inverse(sm, cur, geo, ln, j, stw);
+      Z::y_inverse(sm, cur, geo, ln, j, stw, rtw);
     }
     __syncthreads();
     const double f = occ[state0 + c];
 #pragma unroll
     for (int p = 0; p < YP; ++p) {
-      double2* row = sm + (y0 + ln + p * LPB) * kZRow;
+      const int y = y0 + ln + p * LPB;
+      double2* row = sm + y * kZRow;
       double2 v[V];
-      Z::line_load(v, row, xo);
-      RegFft<N>::template run<true>(v, row, 1, j, stw);
+      Z::line_load(v, sm, y, xo);
+      Z::template fft<true>(v, typename Z::LineAddr{row}, j, stw, rtw);
 #pragma unroll
       for (int t = 0; t < V; ++t)
         acc[p][t] += f * (v[t].x * v[t].x + v[t].y * v[t].y);
@@ -307,7 +404,9 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_density_kernel(
 }
 
 template <int N, int CP>
-__global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_vpsi_kernel(
+__global__ __launch_bounds__(ZCfg<N>::BLOCK,
+                            ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
+zplane_vpsi_kernel(
     double2* __restrict__ R, const double* __restrict__ vpot, int nstates,
     BoxGeom geo, const double2* __restrict__ tw) {
   using Z = ZPlane<N, CP>;
@@ -322,11 +421,11 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_vpsi_kernel(
   for (int i = tid; i < N; i += BLOCK) stw[i] = tw[i];
   __syncthreads();
   if (cb >= ce) return;
-  int ri[CP][V];
+  typename Z::Packed ri[CP], xo;
   Z::row_table(ri, geo, ln, j);
-  int xo[V];
-#pragma unroll
-  for (int m = 0; m < V; ++m) xo[m] = geo.xmap[j + 8 * m];
+  Z::x_table(xo, geo, j);
+  typename Z::Tw rtw;
+  rtw.init(stw, j);
   const double* vz = vpot + (int64_t)z * N * N;
   double vr[Z::VREG ? YP : 1][V];
   if constexpr (Z::VREG) {
@@ -353,15 +452,15 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_vpsi_kernel(
       } else {
         Z::load(cur, ri, buf);
       }
-      Z::y_inverse(sm, cur, geo, ln, j, stw);
+      Z::y_inverse(sm, cur, geo, ln, j, stw, rtw);
     }
     __syncthreads();
-#pragma unroll 1
-    for (int p = 0; p < YP; ++p) {
-      double2* row = sm + (ln + p * LPB) * kZRow;
+    auto line_pass = [&](int p) {
+      const int y = ln + p * LPB;
+      double2* row = sm + y * kZRow;
       double2 v[V];
-      Z::line_load(v, row, xo);
-      RegFft<N>::template run<true>(v, row, 1, j, stw);
+      Z::line_load(v, sm, y, xo);
+      Z::template fft<true>(v, typename Z::LineAddr{row}, j, stw, rtw);
       if constexpr (Z::VREG) {
 #pragma unroll
         for (int t = 0; t < V; ++t) v[t] = cscale(v[t], vr[p][t]);
@@ -370,10 +469,26 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_vpsi_kernel(
 #pragma unroll
         for (int t = 0; t < V; ++t) v[t] = cscale(v[t], vl[8 * t]);
       }
-      RegFft<N>::template run<false>(v, row, 1, j, stw);
+      Z::template fft<false>(v, typename Z::LineAddr{row}, j, stw, rtw);
+      // (branch-free form: off-mask x go to the row's pad slot kZRow - 1,
+      // which nothing reads)
 #pragma unroll
-      for (int t = 0; t < V; ++t)
-        if (xo[t] >= 0) row[xo[t]] = v[t];
+      for (int t = 0; t < V; ++t) {
+        const int x = xo[t];
+        if (JRX_ZBRANCHFREE)
+          sm[x < 0 ? y * kZRow + kZRow - 1 : Z::at(y, x)] = v[t];
+        else if (x >= 0)
+          sm[Z::at(y, x)] = v[t];
+      }
+    };
+    // v(r) in registers needs compile-time pass indices; the 128-point
+    // form keeps the pass loop rolled (register pressure)
+    if constexpr (Z::VREG) {
+#pragma unroll
+      for (int p = 0; p < YP; ++p) line_pass(p);
+    } else {
+#pragma unroll 1
+      for (int p = 0; p < YP; ++p) line_pass(p);
     }
     __syncthreads();
     // forward y transform of the active columns, back to the rows of R
@@ -385,11 +500,15 @@ __global__ __launch_bounds__(ZCfg<N>::BLOCK) void zplane_vpsi_kernel(
       if (xa < geo.nxa) {
         double2 v[V];
 #pragma unroll
-        for (int m = 0; m < V; ++m) v[m] = sm[(j + 8 * m) * kZRow + xa];
-        RegFft<N>::template run<false>(v, sm + xa, kZRow, j, stw);
+        for (int m = 0; m < V; ++m) v[m] = sm[Z::at(j + 8 * m, xa)];
+        Z::template fft<false>(v, typename Z::ColAddr{sm, xa}, j, stw, rtw);
+        // off-mask rows go to the slice's pad element (index nrows)
 #pragma unroll
-        for (int t = 0; t < V; ++t)
-          if (ri[p][t] >= 0) nt_store(buf + ri[p][t], v[t]);
+        for (int t = 0; t < V; ++t) {
+          const int r = ri[p][t];
+          if (JRX_ZBRANCHFREE) nt_store(buf + (unsigned)(r < 0 ? geo.nrows : r), v[t]);
+          else if (r >= 0) nt_store(buf + r, v[t]);
+        }
       }
     }
   }
