@@ -675,7 +675,7 @@ int jrx_ewald(const double* positions, const double* charges, int natom,
 // ------------------------------------------------------ stand-alone QR --
 static size_t qr_ws_layout(int64_t P, int ng, int nb, size_t* o_part,
                            size_t* o_s, size_t* o_a, size_t* o_b) {
-  const GramSplit sp = gram_split(P, ng);
+  const GramSplit sp = gram_split(P, ng, nb);
   const size_t mat = (size_t)P * nb * nb * sizeof(double2);
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -703,7 +703,7 @@ int jrx_qr(const double* w_re, const double* w_im, int64_t P, int ng, int nb,
               "invalid qr arguments (need ng >= nb)");
   JRX_REQUIRE(P <= 65535, "too many QR problems");
   if (nb > kMaxBands) {
-    set_error("nb above 64 is not supported by this build");
+    set_error("nb above 128 is not supported by this build");
     return JRX_ERR_UNSUPPORTED;
   }
   size_t op, os, oa, ob;
@@ -715,13 +715,14 @@ int jrx_qr(const double* w_re, const double* w_im, int64_t P, int ng, int nb,
   double2* S = (double2*)(ws + os);
   double2* R1 = (double2*)(ws + oa);
   double2* R2 = (double2*)(ws + ob);
-  const GramSplit sp = gram_split(P, ng);
+  const GramSplit sp = gram_split(P, ng, nb);
   hipStream_t s = (hipStream_t)stream;
   int rc;
   if ((rc = launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, sp, s)) ||
       (rc = launch_reduce_partials(part, S, P, sp.nchunk, nb, s)) ||
       (rc = launch_chol_inv(S, R1, P, nb, s)) ||
-      (rc = launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, sp, s)) ||
+      (rc = launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, sp, s, nullptr,
+                               (double2*)q)) ||
       (rc = launch_reduce_partials(part, S, P, sp.nchunk, nb, s)) ||
       (rc = launch_chol_inv(S, R2, P, nb, s)) ||
       (rc = launch_trmm_uu(R1, R2, (double2*)rinv, P, nb, s)) ||
@@ -739,7 +740,7 @@ int jrx_qr_backward(const double* q, const double* qbar, const double* rinv,
               "invalid qr_backward arguments");
   JRX_REQUIRE(P <= 65535, "too many QR problems");
   if (nb > kMaxBands) {
-    set_error("nb above 64 is not supported by this build");
+    set_error("nb above 128 is not supported by this build");
     return JRX_ERR_UNSUPPORTED;
   }
   size_t op, os, oa, ob;
@@ -751,7 +752,7 @@ int jrx_qr_backward(const double* q, const double* qbar, const double* rinv,
   double2* G = (double2*)(ws + os);
   double2* A = (double2*)(ws + oa);
   double2* Bm = (double2*)(ws + ob);
-  const GramSplit sp = gram_split(P, ng);
+  const GramSplit sp = gram_split(P, ng, nb);
   hipStream_t s = (hipStream_t)stream;
   int rc;
   if ((rc = launch_gram_bmajor((const double2*)q, (const double2*)qbar, part,

@@ -194,7 +194,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     return JRX_ERR_UNSUPPORTED;
   }
   if (d->nb > kMaxBands) {
-    set_error("nb above 64 is not supported by this build");
+    set_error("nb above 128 is not supported by this build");
     return JRX_ERR_UNSUPPORTED;
   }
   JRX_REQUIRE(d->fft_variant >= 0 && d->fft_variant <= 2,
@@ -330,12 +330,10 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   // (the row path's largest block holds an nz x 72 z-plane and twiddles)
   const size_t lds_plane =
       (p->rowpath ? (size_t)nz * 73 : (size_t)ny * nz) * sizeof(double2);
-  const size_t nbp = 16 * (size_t)((nb + 15) / 16);
-  const size_t lds_qr = 2 * nbp * nbp * sizeof(double2);
-  const size_t lds_gram = 2 * nbp * nbp * sizeof(double2);  // pre + sums
-  const size_t lds_chol = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
+  // (the output transpose of jrx_plan_orthonormal_params: [nb][65])
+  const size_t lds_params = (size_t)nb * 65 * sizeof(double2);
   size_t need = lds_plane;
-  for (size_t v : {lds_qr, lds_gram, lds_chol}) need = v > need ? v : need;
+  for (size_t v : {qr_lds_bytes(nb), lds_params}) need = v > need ? v : need;
   if (need > (size_t)max_lds) {
     set_error("problem needs " + std::to_string(need) +
               " bytes of LDS per workgroup; device allows " +
@@ -389,7 +387,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     p->cpz.B[i] = p->B[6 + i];
     p->cpz.B[6 + i] = p->B[i];
   }
-  p->gsw = gram_split(p->P, ng);
+  p->gsw = gram_split(p->P, ng, nb);
   p->gsq = p->gsw;
   const int64_t kin_gather =
       p->S * (p->rowpath ? p->row_chunks : p->geo.nxa);
@@ -725,25 +723,27 @@ extern "C" int jrx_plan_set_occupation(jrx_plan* p, const double* occupation,
 }
 
 // Q [P][nb][ng] (complex, band-major) -> out_re / out_im [P][ng][nb], one
-// 64-wide G tile of one (s,k) block per workgroup, transposed through LDS.
+// 64-wide G tile of one (s,k) block per workgroup, transposed through LDS
+// (tile [nb][65], dynamic).
 __global__ void __launch_bounds__(256)
 q_to_params_kernel(const double2* __restrict__ q, int ng, int nb,
                    double* __restrict__ outr, double* __restrict__ outi) {
-  __shared__ double2 tile[kMaxBands][65];
+  extern __shared__ double2 tile[];
   const int64_t pk = blockIdx.y;
   const int g0 = blockIdx.x * 64;
   const double2* src = q + pk * (int64_t)nb * ng;
   for (int i = threadIdx.x; i < nb * 64; i += 256) {
     const int b = i >> 6, gl = i & 63, g = g0 + gl;
-    tile[b][gl] = g < ng ? src[(int64_t)b * ng + g] : make_double2(0.0, 0.0);
+    tile[b * 65 + gl] =
+        g < ng ? src[(int64_t)b * ng + g] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb * 64; i += 256) {
     const int gl = i / nb, b = i % nb, g = g0 + gl;
     if (g < ng) {
       const int64_t o = (pk * ng + g) * nb + b;
-      outr[o] = tile[b][gl].x;
-      outi[o] = tile[b][gl].y;
+      outr[o] = tile[b * 65 + gl].x;
+      outi[o] = tile[b * 65 + gl].y;
     }
   }
 }
@@ -759,37 +759,43 @@ extern "C" int jrx_plan_orthonormal_params(const jrx_plan* p, double* out_re,
       static_cast<const char*>(workspace) + p->o_q);
   hipLaunchKernelGGL(q_to_params_kernel,
                      dim3((unsigned)ceil_div(p->ng, 64), (unsigned)p->P),
-                     dim3(256), 0, (hipStream_t)stream, q, p->ng, p->nb,
-                     out_re, out_im);
+                     dim3(256), (size_t)p->nb * 65 * sizeof(double2),
+                     (hipStream_t)stream, q, p->ng, p->nb, out_re, out_im);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
 
 // out[p][g][b] = sum_c Q[p][c][g] V[p][c][b]: one 32-wide G tile of one
-// (s,k) block per workgroup; the Q tile and V[p] are staged in LDS.
+// (s,k) block per workgroup; the Q tile ([nb][kRotG + 1]) and, up to 64
+// bands, V[p] are staged in LDS (dynamic); wider V is read through the
+// caches.
 constexpr int kRotG = 32;
 __global__ void __launch_bounds__(256)
 q_rotate_kernel(const double2* __restrict__ q, const double2* __restrict__ V,
                 int ng, int nb, double* __restrict__ outr,
                 double* __restrict__ outi) {
-  __shared__ double2 tile[kMaxBands][kRotG + 1];
-  __shared__ double2 vs[kMaxBands * kMaxBands];
+  extern __shared__ double2 tile[];
+  const bool vlds = nb <= kNarrowBands;
+  double2* vs = tile + nb * (kRotG + 1);
   const int64_t pk = blockIdx.y;
   const int g0 = blockIdx.x * kRotG;
   const double2* src = q + pk * (int64_t)nb * ng;
   for (int i = threadIdx.x; i < nb * kRotG; i += 256) {
     const int c = i / kRotG, gl = i % kRotG, g = g0 + gl;
-    tile[c][gl] = g < ng ? src[(int64_t)c * ng + g] : make_double2(0.0, 0.0);
+    tile[c * (kRotG + 1) + gl] =
+        g < ng ? src[(int64_t)c * ng + g] : make_double2(0.0, 0.0);
   }
   const double2* vp = V + pk * nb * nb;
-  for (int i = threadIdx.x; i < nb * nb; i += 256) vs[i] = vp[i];
+  if (vlds)
+    for (int i = threadIdx.x; i < nb * nb; i += 256) vs[i] = vp[i];
   __syncthreads();
+  const double2* vsrc = vlds ? vs : vp;
   for (int i = threadIdx.x; i < nb * kRotG; i += 256) {
     const int gl = i / nb, b = i % nb, g = g0 + gl;
     if (g >= ng) continue;
     double re = 0.0, im = 0.0;
     for (int c = 0; c < nb; ++c) {
-      const double2 a = tile[c][gl], v = vs[c * nb + b];
+      const double2 a = tile[c * (kRotG + 1) + gl], v = vsrc[c * nb + b];
       re += a.x * v.x - a.y * v.y;
       im += a.x * v.y + a.y * v.x;
     }
@@ -819,9 +825,13 @@ extern "C" int jrx_plan_rotate(const jrx_plan* p, const double* v,
   JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
   const double2* q = reinterpret_cast<const double2*>(
       static_cast<const char*>(workspace) + p->o_q);
+  const size_t shm =
+      ((size_t)p->nb * (kRotG + 1) +
+       (p->nb <= kNarrowBands ? (size_t)p->nb * p->nb : 0)) *
+      sizeof(double2);
   hipLaunchKernelGGL(q_rotate_kernel,
                      dim3((unsigned)ceil_div(p->ng, kRotG), (unsigned)p->P),
-                     dim3(256), 0, (hipStream_t)stream, q,
+                     dim3(256), shm, (hipStream_t)stream, q,
                      reinterpret_cast<const double2*>(v), p->ng, p->nb,
                      out_re, out_im);
   JRX_LAUNCH_CHECK();
@@ -1100,7 +1110,7 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, nullptr, part, P, ng, nb, p->gsw, s));
   JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s, one_pass));
-  JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s, one_pass));
+  JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s, one_pass, q));
   JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s, nullptr, one_pass));
   JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));

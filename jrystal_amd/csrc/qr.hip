@@ -13,11 +13,16 @@
 //   copyltu(X) = X + X^H - Re diag(X).
 #include "qr_kernels.h"
 
+#include <algorithm>
+
 namespace jrx {
 
-GramSplit gram_split(int64_t P, int ng) {
+GramSplit gram_split(int64_t P, int ng, int nb) {
   const int tiles = (ng + kQrRows - 1) / kQrRows;
-  int64_t want = (2048 + P - 1) / P;  // chunks per problem
+  // about 2048 blocks in all; the wide kernels do four to sixteen times
+  // the work per row, and each chunk adds an nb x nb partial to reduce
+  const int64_t target = nb > kNarrowBands ? 512 : 2048;
+  int64_t want = (target + P - 1) / P;  // chunks per problem
   if (want < 1) want = 1;
   if (want > tiles) want = tiles;
   const int tiles_per_chunk = (int)((tiles + want - 1) / want);
@@ -46,6 +51,397 @@ __device__ __forceinline__ void load_small(double2* dst, const double2* src,
     const int i = e / NBP, j = e - i * NBP;
     dst[e] = (i < nb && j < nb) ? src[i * nb + j] : czero();
   }
+}
+
+// ------------------------------------------------- wide kernels (nb > 64) --
+// Tile pairs (I, J) of the wide Gram kernels: HERM takes I <= J (row-major
+// over the upper triangle; the result is Hermitian), otherwise all T x T.
+// NZ blocks share one chunk's pairs, PW pairs to a wave.
+template <int T, bool HERM>
+struct WidePairs {
+  static constexpr int NPAIR = HERM ? T * (T + 1) / 2 : T * T;
+  static constexpr int NZ = (NPAIR + 35) / 36;
+  static constexpr int PW = (NPAIR + 4 * NZ - 1) / (4 * NZ);
+};
+
+__device__ inline void decode_pair(int q, int T, bool herm, int* I, int* J) {
+  if (!herm) {
+    *I = q / T;
+    *J = q - *I * T;
+    return;
+  }
+  int i = 0;
+  while (q >= T - i) {
+    q -= T - i;
+    ++i;
+  }
+  *I = i;
+  *J = i + q;
+}
+
+constexpr int kWideSlab = 16;  // rows per LDS slab of the wide Gram kernel
+
+// Partial Gram sums over a chunk's rows, one partial per chunk:
+//   GMAJOR: S = W^H W (W planar g-major, the Hermitian pairs only);
+//   else:   G = A^H B (A, B band-major complex; HERM when B is A).
+// Row slabs are staged in LDS ([row][NBP + 1]); each wave accumulates its
+// own tile pairs and writes them (and, for HERM, their mirror images).
+template <int T, bool GMAJOR, bool HERM>
+__global__ __launch_bounds__(kQrBlock) void gram_wide_kernel(
+    const double* __restrict__ wr, const double* __restrict__ wi,
+    const double2* __restrict__ qa, const double2* __restrict__ qb,
+    double2* __restrict__ partial, int ng, int nb, int rows_per_chunk,
+    int nchunk, const int* __restrict__ skip) {
+  using PL = WidePairs<T, HERM>;
+  constexpr int NBP = 16 * T, LD = NBP + 1, PW = PL::PW, SR = kWideSlab;
+  extern __shared__ double2 sm[];
+  double2* sa = sm;
+  double2* sb = HERM ? sm : sm + SR * LD;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t p = blockIdx.y;
+  const int chunk = blockIdx.x;
+  if (skip && skip[p]) return;  // uniform over the block
+  int pI[PW], pJ[PW];
+  bool live[PW];
+#pragma unroll
+  for (int t = 0; t < PW; ++t) {
+    const int q = ((int)blockIdx.z * 4 + wave) * PW + t;
+    live[t] = q < PL::NPAIR;  // uniform over the wave
+    decode_pair(live[t] ? q : 0, T, HERM, &pI[t], &pJ[t]);
+  }
+  d4 re[PW], im[PW];
+#pragma unroll
+  for (int t = 0; t < PW; ++t) { re[t] = d4zero(); im[t] = d4zero(); }
+  const int gb = chunk * rows_per_chunk;
+  const int ge = min(ng, gb + rows_per_chunk);
+  const double* Wr = wr + p * ng * nb;
+  const double* Wi = wi + p * ng * nb;
+  const double2* A = qa + p * nb * ng;
+  const double2* B = qb + p * nb * ng;
+  for (int g0 = gb; g0 < ge; g0 += SR) {
+    __syncthreads();  // the previous slab is consumed
+    for (int e = threadIdx.x; e < SR * NBP; e += kQrBlock) {
+      if (GMAJOR) {  // consecutive threads: consecutive bands of a row
+        const int r = e / NBP, c = e - r * NBP, g = g0 + r;
+        const bool ok = g < ge && c < nb;
+        sa[r * LD + c] = ok ? make_double2(Wr[(int64_t)g * nb + c],
+                                           Wi[(int64_t)g * nb + c])
+                            : czero();
+      } else {  // consecutive threads: consecutive rows of a band
+        const int c = e / SR, r = e - c * SR, g = g0 + r;
+        const bool ok = g < ge && c < nb;
+        sa[r * LD + c] = ok ? A[(int64_t)c * ng + g] : czero();
+        if (!HERM) sb[r * LD + c] = ok ? B[(int64_t)c * ng + g] : czero();
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SR / 4; ++r) {
+      const double2* ra = sa + (4 * r + lk) * LD + li;
+      const double2* rb = sb + (4 * r + lk) * LD + li;
+#pragma unroll
+      for (int t = 0; t < PW; ++t) {
+        if (!live[t]) continue;
+        const double2 a = ra[16 * pI[t]], b = rb[16 * pJ[t]];
+        // conj(a) b
+        re[t] = mfma4(a.x, b.x, re[t]);
+        re[t] = mfma4(a.y, b.y, re[t]);
+        im[t] = mfma4(a.x, b.y, im[t]);
+        im[t] = mfma4(-a.y, b.x, im[t]);
+      }
+    }
+  }
+  double2* out = partial + (p * nchunk + chunk) * nb * nb;
+#pragma unroll
+  for (int t = 0; t < PW; ++t) {
+    if (!live[t]) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * pI[t] + 4 * r + lk, j = 16 * pJ[t] + li;
+      if (i < nb && j < nb) {
+        const double2 v = make_double2(re[t][r], im[t][r]);
+        out[i * nb + j] = v;
+        if (HERM && pI[t] < pJ[t]) out[j * nb + i] = cconj(v);
+      }
+    }
+  }
+}
+
+// GMAJOR when wr is given (qa, qb unused), else band-major qa, qb (HERM when
+// qa == qb).
+int launch_gram_wide(const double* wr, const double* wi, const double2* qa,
+                     const double2* qb, double2* partial, int64_t P, int ng,
+                     int nb, const GramSplit& sp, hipStream_t s,
+                     const int* skip) {
+  const int T = (nb + 15) / 16;
+  const bool gmajor = wr != nullptr, herm = gmajor || qa == qb;
+  const size_t shm =
+      (herm ? 1 : 2) * (size_t)kWideSlab * (16 * T + 1) * sizeof(double2);
+  dim3 block(kQrBlock);
+#define JRX_GRAM_WIDE(TT, GM, HM)                                            \
+  hipLaunchKernelGGL((gram_wide_kernel<TT, GM, HM>),                         \
+                     dim3(sp.nchunk, (unsigned)P, WidePairs<TT, HM>::NZ),    \
+                     block, shm, s, wr, wi, qa, qb, partial, ng, nb,         \
+                     sp.rows_per_chunk, sp.nchunk, skip)
+#define JRX_GRAM_WIDE_T(TT)                              \
+  if (gmajor) JRX_GRAM_WIDE(TT, true, true);             \
+  else if (herm) JRX_GRAM_WIDE(TT, false, true);         \
+  else JRX_GRAM_WIDE(TT, false, false);
+  switch (T) {
+    case 5: JRX_GRAM_WIDE_T(5) break;
+    case 6: JRX_GRAM_WIDE_T(6) break;
+    case 7: JRX_GRAM_WIDE_T(7) break;
+    case 8: JRX_GRAM_WIDE_T(8) break;
+    default:
+      set_error("nb above 128 is not supported by this build");
+      return JRX_ERR_UNSUPPORTED;
+  }
+#undef JRX_GRAM_WIDE_T
+#undef JRX_GRAM_WIDE
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
+constexpr int kCholWideBlock = 512;
+
+// Element (i, j), i <= j, of an upper triangle packed row by row.
+__device__ inline int up_index(int i, int j, int nb) {
+  return i * nb - (i * (i - 1)) / 2 + (j - i);
+}
+
+// chol_inv_kernel for nb > 64: the upper triangle, packed, is the only
+// matrix in LDS (132 KB at nb = 128); R is factored in place and then
+// inverted in place column by column,
+//   X[j][j] = 1 / R[j][j],  X[i][j] = -sum_{k=i}^{j-1} X[i][k] R[k][j] / R[j][j],
+// with column j of R copied aside first.
+__global__ __launch_bounds__(kCholWideBlock) void chol_inv_wide_kernel(
+    const double2* __restrict__ S, double2* __restrict__ rinv, int nb,
+    int* __restrict__ flag_out, const int* __restrict__ flag_in) {
+  extern __shared__ double2 sm[];
+  const int np = nb * (nb + 1) / 2;
+  double2* A = sm;
+  double2* col = sm + np;
+  double* red = reinterpret_cast<double*>(col + nb);
+  const int64_t p = blockIdx.x;
+  const int tid = threadIdx.x;
+  constexpr int B = kCholWideBlock;
+  double2* out = rinv + p * nb * nb;
+  if (flag_in && flag_in[p]) {  // uniform over the block
+    for (int e = tid; e < nb * nb; e += B) {
+      const int i = e / nb, j = e - i * nb;
+      out[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    return;
+  }
+  const double2* Sp = S + p * nb * nb;
+  for (int e = tid; e < nb * nb; e += B) {
+    const int i = e / nb, j = e - i * nb;
+    if (j >= i) A[up_index(i, j, nb)] = Sp[e];
+  }
+  __syncthreads();
+  if (flag_out) {  // distance of the scaled Gram matrix from I
+    double a = 0.0;
+    for (int e = tid; e < nb * nb; e += B) {
+      const int i = e / nb, j = e - i * nb;
+      if (i < j) {  // (i, j) and (j, i)
+        const double2 v = A[up_index(i, j, nb)];
+        a += 2.0 * (v.x * v.x + v.y * v.y) /
+             (A[up_index(i, i, nb)].x * A[up_index(j, j, nb)].x);
+      }
+    }
+    const double d2 = block_sum(a, red);
+    if (tid == 0) flag_out[p] = (d2 < kQrOnePass * kQrOnePass) ? 1 : 0;
+  }
+  for (int j = 0; j < nb; ++j) {
+    const int jj = up_index(j, j, nb);
+    if (tid == 0) {
+      const double d = A[jj].x;
+      // NaN for a non-positive pivot: sqrt of a negative number.
+      A[jj] = make_double2(d > 0.0 ? sqrt(d) : sqrt(-1.0 - d * 0.0), 0.0);
+    }
+    __syncthreads();
+    const double d = A[jj].x;
+    for (int c = j + 1 + tid; c < nb; c += B) A[jj + c - j] = cscale(A[jj + c - j], 1.0 / d);
+    __syncthreads();
+    const int m = nb - j - 1;
+    for (int e = tid; e < m * m; e += B) {
+      const int r = j + 1 + e / m, c = j + 1 + e % m;
+      if (c >= r)
+        A[up_index(r, c, nb)] =
+            csub(A[up_index(r, c, nb)], cmulc(A[jj + r - j], A[jj + c - j]));
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < nb; ++j) {
+    for (int k = tid; k <= j; k += B) col[k] = A[up_index(k, j, nb)];
+    __syncthreads();
+    const double rjj = col[j].x;
+    for (int i = tid; i < j; i += B) {
+      double2 acc = czero();
+      for (int k = i; k < j; ++k) acc = cadd(acc, cmul(A[up_index(i, k, nb)], col[k]));
+      A[up_index(i, j, nb)] = cscale(acc, -1.0 / rjj);
+    }
+    if (tid == 0) A[up_index(j, j, nb)] = make_double2(1.0 / rjj, 0.0);
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * nb; e += B) {
+    const int i = e / nb, j = e - i * nb;
+    out[e] = (j >= i) ? A[up_index(i, j, nb)] : czero();
+  }
+}
+
+// qr_bwd_small_kernel for nb > 64: one output entry per thread, operands
+// read from global memory (L2-resident nb x nb matrices).
+__global__ __launch_bounds__(kQrBlock) void qr_bwd_small_wide_kernel(
+    const double2* __restrict__ G, const double2* __restrict__ rinv,
+    double2* __restrict__ A, double2* __restrict__ Bm, int nb) {
+  const int64_t p = blockIdx.y;
+  const int e = blockIdx.x * kQrBlock + threadIdx.x;
+  if (e >= nb * nb) return;
+  const int i = e / nb, j = e - i * nb;
+  const double2* Gp = G + p * nb * nb;
+  const double2* Rp = rinv + p * nb * nb;
+  A[p * nb * nb + e] = cconj(Rp[j * nb + i]);
+  double2 acc = czero();
+  for (int k = j; k < nb; ++k) {
+    double2 m;  // copyltu(triu(-G))[i][k]
+    if (i < k) m = make_double2(-Gp[i * nb + k].x, -Gp[i * nb + k].y);
+    else if (i > k) m = make_double2(-Gp[k * nb + i].x, Gp[k * nb + i].y);
+    else m = make_double2(-Gp[i * nb + i].x, 0.0);
+    acc = cadd(acc, cmul(m, cconj(Rp[j * nb + k])));
+  }
+  Bm[p * nb * nb + e] = acc;
+}
+
+constexpr int kWideRows = 64;  // g rows per block of the wide apply kernels
+constexpr int kWideK = 32;     // rinv rows per LDS chunk (apply_fwd)
+constexpr int kWideKB = 16;    // band rows per LDS chunk (apply_bwd)
+
+// apply_fwd_kernel for nb > 64: rinv staged in chunks of kWideK rows, one
+// 16-row slab of W per wave; tiles below the diagonal of rinv are skipped.
+template <int T>
+__global__ __launch_bounds__(kQrBlock) void apply_fwd_wide_kernel(
+    const double* __restrict__ wr, const double* __restrict__ wi,
+    const double2* __restrict__ rinv, double2* __restrict__ q, int ng, int nb,
+    const int* __restrict__ skip) {
+  extern __shared__ double2 sm[];  // [kWideK][NBP]
+  constexpr int NBP = 16 * T;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t p = blockIdx.y;
+  if (skip && skip[p]) return;  // uniform over the block
+  const double2* Rp = rinv + p * nb * nb;
+  const double* Wr = wr + p * ng * nb;
+  const double* Wi = wi + p * ng * nb;
+  double2* Q = q + p * nb * ng;
+  const int g0 = blockIdx.x * kWideRows + wave * 16;
+  const int row = g0 + li;
+  d4 qre[T], qim[T];
+#pragma unroll
+  for (int J = 0; J < T; ++J) { qre[J] = d4zero(); qim[J] = d4zero(); }
+  for (int k0 = 0; k0 < nb; k0 += kWideK) {
+    __syncthreads();  // the previous chunk is consumed
+    for (int e = threadIdx.x; e < kWideK * NBP; e += kQrBlock) {
+      const int kk = e / NBP, b = e - kk * NBP, k = k0 + kk;
+      sm[e] = (k < nb && b < nb) ? Rp[k * nb + b] : czero();
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int m = 0; m < kWideK / 4; ++m) {
+      const int col = k0 + 4 * m + lk;
+      const bool ok = row < ng && col < nb;
+      const double a_r = ok ? Wr[(int64_t)row * nb + col] : 0.0;
+      const double a_i = ok ? Wi[(int64_t)row * nb + col] : 0.0;
+#pragma unroll
+      for (int J = 0; J < T; ++J) {
+        if (16 * J + 15 < k0) continue;  // rinv[k][b] = 0 for k > b
+        const double2 rv = sm[(4 * m + lk) * NBP + 16 * J + li];
+        qre[J] = mfma4(rv.x, a_r, qre[J]);
+        qre[J] = mfma4(-rv.y, a_i, qre[J]);
+        qim[J] = mfma4(rv.x, a_i, qim[J]);
+        qim[J] = mfma4(rv.y, a_r, qim[J]);
+      }
+    }
+  }
+#pragma unroll
+  for (int J = 0; J < T; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int b = 16 * J + 4 * r + lk;
+      if (b < nb && row < ng)
+        Q[(int64_t)b * ng + row] = make_double2(qre[J][r], qim[J][r]);
+    }
+}
+
+// apply_bwd_kernel for nb > 64: A and Bm staged in chunks of kWideKB band
+// rows, one 16-row slab of the output per wave; A = rinv^H is lower
+// triangular, so its tiles right of the chunk are skipped.
+template <int T>
+__global__ __launch_bounds__(kQrBlock) void apply_bwd_wide_kernel(
+    const double2* __restrict__ qbar, const double2* __restrict__ q,
+    const double2* __restrict__ A, const double2* __restrict__ Bm,
+    double* __restrict__ dwr, double* __restrict__ dwi, int ng, int nb) {
+  extern __shared__ double2 sm[];
+  constexpr int NBP = 16 * T;
+  double2* sa = sm;                   // [kWideKB][NBP]
+  double2* sb = sm + kWideKB * NBP;   // [kWideKB][NBP]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t p = blockIdx.y;
+  const double2* Ap = A + p * nb * nb;
+  const double2* Bp = Bm + p * nb * nb;
+  const double2* Qb = qbar + p * nb * ng;
+  const double2* Q = q + p * nb * ng;
+  double* Dr = dwr + p * ng * nb;
+  double* Di = dwi + p * ng * nb;
+  const int g0 = blockIdx.x * kWideRows + wave * 16;
+  const int g = g0 + li;
+  d4 re[T], im[T];
+#pragma unroll
+  for (int J = 0; J < T; ++J) { re[J] = d4zero(); im[J] = d4zero(); }
+  for (int k0 = 0; k0 < nb; k0 += kWideKB) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kWideKB * NBP; e += kQrBlock) {
+      const int kk = e / NBP, b = e - kk * NBP, k = k0 + kk;
+      const bool ok = k < nb && b < nb;
+      sa[e] = ok ? Ap[k * nb + b] : czero();
+      sb[e] = ok ? Bp[k * nb + b] : czero();
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int m = 0; m < kWideKB / 4; ++m) {
+      const int band = k0 + 4 * m + lk;
+      const bool ok = g < ng && band < nb;
+      const double2 vb = ok ? Qb[(int64_t)band * ng + g] : czero();
+      const double2 vq = ok ? Q[(int64_t)band * ng + g] : czero();
+#pragma unroll
+      for (int J = 0; J < T; ++J) {
+        if (16 * J < k0 + kWideKB) {  // A[i][j] = 0 for j > i
+          const double2 a = sa[(4 * m + lk) * NBP + 16 * J + li];
+          re[J] = mfma4(vb.x, a.x, re[J]);
+          re[J] = mfma4(-vb.y, a.y, re[J]);
+          im[J] = mfma4(vb.x, a.y, im[J]);
+          im[J] = mfma4(vb.y, a.x, im[J]);
+        }
+        const double2 b = sb[(4 * m + lk) * NBP + 16 * J + li];
+        re[J] = mfma4(vq.x, b.x, re[J]);
+        re[J] = mfma4(-vq.y, b.y, re[J]);
+        im[J] = mfma4(vq.x, b.y, im[J]);
+        im[J] = mfma4(vq.y, b.x, im[J]);
+      }
+    }
+  }
+#pragma unroll
+  for (int J = 0; J < T; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = g0 + 4 * r + lk, col = 16 * J + li;
+      if (row < ng && col < nb) {
+        Dr[(int64_t)row * nb + col] = re[J][r];
+        Di[(int64_t)row * nb + col] = im[J][r];
+      }
+    }
 }
 
 // Partial Gram sums over the chunk's rows (one per block):
@@ -161,8 +557,21 @@ __global__ __launch_bounds__(kQrBlock) void gram_w_kernel(
 
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
-                       const GramSplit& sp, hipStream_t s, const int* skip) {
+                       const GramSplit& sp, hipStream_t s, const int* skip,
+                       double2* q1) {
   const int T = (nb + 15) / 16;
+  if (nb > kNarrowBands) {
+    if (!pre) return launch_gram_wide(wr, wi, nullptr, nullptr, partial, P, ng,
+                                      nb, sp, s, skip);
+    if (!q1) {
+      set_error("the wide Gram pass with a preconditioner needs q1");
+      return JRX_ERR_INVALID;
+    }
+    const int rc = launch_apply_fwd(wr, wi, pre, q1, P, ng, nb, s, skip);
+    if (rc) return rc;
+    return launch_gram_wide(nullptr, nullptr, q1, q1, partial, P, ng, nb, sp,
+                            s, skip);
+  }
   size_t shm = (pre ? 2 : 1) * (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
 #define JRX_GRAM_W(TT)                                                        \
@@ -261,6 +670,9 @@ int launch_gram_bmajor(const double2* q, const double2* qb, double2* partial,
                        int64_t P, int ng, int nb, const GramSplit& sp,
                        hipStream_t s) {
   const int T = (nb + 15) / 16;
+  if (nb > kNarrowBands)
+    return launch_gram_wide(nullptr, nullptr, q, qb, partial, P, ng, nb, sp, s,
+                            nullptr);
   const size_t shm = (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(sp.nchunk, (unsigned)P), block(kQrBlock);
   switch (T) {
@@ -371,6 +783,15 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
 
 int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
                     hipStream_t s, int* flag_out, const int* flag_in) {
+  if (nb > kNarrowBands) {
+    const size_t shm = ((size_t)nb * (nb + 1) / 2 + nb) * sizeof(double2) +
+                       (kCholWideBlock / 64) * sizeof(double);
+    hipLaunchKernelGGL(chol_inv_wide_kernel, dim3((unsigned)P),
+                       dim3(kCholWideBlock), shm, s, S, rinv, nb, flag_out,
+                       flag_in);
+    JRX_LAUNCH_CHECK();
+    return JRX_OK;
+  }
   size_t shm = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
   hipLaunchKernelGGL(chol_inv_kernel, dim3((unsigned)P), dim3(kQrBlock), shm,
                      s, S, rinv, nb, flag_out, flag_in);
@@ -435,6 +856,13 @@ __global__ __launch_bounds__(kQrBlock) void qr_bwd_small_kernel(
 
 int launch_qr_bwd_small(const double2* G, const double2* rinv, double2* A,
                         double2* Bm, int64_t P, int nb, hipStream_t s) {
+  if (nb > kNarrowBands) {
+    hipLaunchKernelGGL(qr_bwd_small_wide_kernel,
+                       dim3(ceil_div(nb * nb, kQrBlock), (unsigned)P),
+                       dim3(kQrBlock), 0, s, G, rinv, A, Bm, nb);
+    JRX_LAUNCH_CHECK();
+    return JRX_OK;
+  }
   size_t shm = 2 * (size_t)nb * (nb + 1) * sizeof(double2);
   hipLaunchKernelGGL(qr_bwd_small_kernel, dim3((unsigned)P), dim3(kQrBlock),
                      shm, s, G, rinv, A, Bm, nb);
@@ -496,8 +924,28 @@ __global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
 }
 
 int launch_apply_fwd(const double* wr, const double* wi, const double2* rinv,
-                     double2* q, int64_t P, int ng, int nb, hipStream_t s) {
+                     double2* q, int64_t P, int ng, int nb, hipStream_t s,
+                     const int* skip) {
   const int T = (nb + 15) / 16;
+  if (nb > kNarrowBands) {
+    const size_t shm = (size_t)kWideK * 16 * T * sizeof(double2);
+    dim3 grid(ceil_div(ng, kWideRows), (unsigned)P), block(kQrBlock);
+    switch (T) {
+      case 5: hipLaunchKernelGGL((apply_fwd_wide_kernel<5>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, skip); break;
+      case 6: hipLaunchKernelGGL((apply_fwd_wide_kernel<6>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, skip); break;
+      case 7: hipLaunchKernelGGL((apply_fwd_wide_kernel<7>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, skip); break;
+      case 8: hipLaunchKernelGGL((apply_fwd_wide_kernel<8>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, skip); break;
+      default:
+        set_error("nb above 128 is not supported by this build");
+        return JRX_ERR_UNSUPPORTED;
+    }
+    JRX_LAUNCH_CHECK();
+    return JRX_OK;
+  }
+  if (skip) {
+    set_error("launch_apply_fwd: skip is for the wide kernels");
+    return JRX_ERR_INVALID;
+  }
   size_t shm = (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(ceil_div(ng, kApplyRows), (unsigned)P), block(kQrBlock);
   switch (T) {
@@ -574,10 +1022,37 @@ __global__ __launch_bounds__(kQrBlock) void apply_bwd_kernel(
   }
 }
 
+size_t qr_lds_bytes(int nb) {
+  const size_t nbp = 16 * (size_t)((nb + 15) / 16), z = sizeof(double2);
+  if (nb <= kNarrowBands)  // Gram (pre + sums), apply_bwd, chol_inv
+    return std::max(2 * nbp * nbp * z, 2 * (size_t)nb * (nb + 1) * z);
+  const size_t gram = 2 * (size_t)kWideSlab * (nbp + 1) * z;
+  const size_t fwd = (size_t)kWideK * nbp * z;
+  const size_t bwd = 2 * (size_t)kWideKB * nbp * z;
+  const size_t chol = ((size_t)nb * (nb + 1) / 2 + nb) * z +
+                      (kCholWideBlock / 64) * sizeof(double);
+  return std::max(std::max(gram, fwd), std::max(bwd, chol));
+}
+
 int launch_apply_bwd(const double2* qbar, const double2* q, const double2* A,
                      const double2* Bm, double* dwr, double* dwi, int64_t P,
                      int ng, int nb, hipStream_t s) {
   const int T = (nb + 15) / 16;
+  if (nb > kNarrowBands) {
+    const size_t shm = 2 * (size_t)kWideKB * 16 * T * sizeof(double2);
+    dim3 grid(ceil_div(ng, kWideRows), (unsigned)P), block(kQrBlock);
+    switch (T) {
+      case 5: hipLaunchKernelGGL((apply_bwd_wide_kernel<5>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+      case 6: hipLaunchKernelGGL((apply_bwd_wide_kernel<6>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+      case 7: hipLaunchKernelGGL((apply_bwd_wide_kernel<7>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+      case 8: hipLaunchKernelGGL((apply_bwd_wide_kernel<8>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+      default:
+        set_error("nb above 128 is not supported by this build");
+        return JRX_ERR_UNSUPPORTED;
+    }
+    JRX_LAUNCH_CHECK();
+    return JRX_OK;
+  }
   size_t shm = 2 * (size_t)(16 * T) * (16 * T) * sizeof(double2);
   dim3 grid(ceil_div(ng, kApplyRows), (unsigned)P), block(kQrBlock);
   switch (T) {

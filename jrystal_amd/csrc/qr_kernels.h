@@ -12,23 +12,31 @@ namespace jrx {
 
 constexpr int kQrBlock = 256;
 constexpr int kQrRows = 64;   // rows per LDS tile
-constexpr int kMaxBands = 64;
+// Up to kNarrowBands bands one wave holds every 16 x 16 tile of a problem
+// in registers; above it (up to kMaxBands) the wide kernels stage row
+// slabs or band chunks in LDS and spread the tiles over the waves.
+constexpr int kNarrowBands = 64;
+constexpr int kMaxBands = 128;
 
 struct GramSplit {
   int rows_per_chunk;  // multiple of kQrRows
   int nchunk;          // chunks per problem
 };
-GramSplit gram_split(int64_t P, int ng);
+GramSplit gram_split(int64_t P, int ng, int nb);
+
+// Largest LDS allocation (bytes per workgroup) of the QR kernels for nb.
+size_t qr_lds_bytes(int nb);
 
 // partial[p][c] = sum over the rows of chunk c of conj(a_g)^T a_g,
 // a_g = W[p][g][:] (times pre[p] on the right when pre != nullptr): one
 // partial per chunk (the block's four waves are summed in LDS).
 // With skip, problems p with skip[p] != 0 are left out (their partials are
-// not written).
+// not written).  Above kNarrowBands with pre != nullptr, A = W pre is first
+// written band-major into q1 ([P][nb][ng] complex scratch).
 int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
                        double2* partial, int64_t P, int ng, int nb,
                        const GramSplit& sp, hipStream_t s,
-                       const int* skip = nullptr);
+                       const int* skip = nullptr, double2* q1 = nullptr);
 
 // partial[p][c] = sum over the rows of chunk c of
 // conj(Q[p][:][g]) (x) Qb[p][:][g] (one partial per chunk).
@@ -57,9 +65,11 @@ int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
 int launch_trmm_uu(const double2* a, const double2* b, double2* c, int64_t P,
                    int nb, hipStream_t s);
 
-// Q[p][b][g] = sum_i W[p][g][i] rinv[p][i][b].
+// Q[p][b][g] = sum_i W[p][g][i] rinv[p][i][b] (rinv upper triangular);
+// problems with skip[p] != 0 are left out.
 int launch_apply_fwd(const double* wr, const double* wi, const double2* rinv,
-                     double2* q, int64_t P, int ng, int nb, hipStream_t s);
+                     double2* q, int64_t P, int ng, int nb, hipStream_t s,
+                     const int* skip = nullptr);
 
 // From G = Q^H Qbar: A = Rinv^H, Bm = copyltu(triu(-G)) Rinv^H.
 int launch_qr_bwd_small(const double2* G, const double2* rinv, double2* A,
