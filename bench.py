@@ -18,11 +18,17 @@ path runs on one GPU), --steps K, --warmup W, --no-cpu-baseline,
 --cpu-seconds S (target length of the CPU baseline sample),
 --no-convergence, --conv-seconds S (time cap of each convergence run),
 --no-batch, --batch-steps K (the uneven-structure batch section),
---traffic-probe (the workload of the rocprofv3 counter runs).
+--traffic-probe (the workload of the rocprofv3 counter runs),
+--no-large-cell, --large-cell-steps K, --large-cell-probe (the 128^3 /
+128-band section and its rocprofv3 workload).
 
 A third section, "batch", times Adam steps (energy + gradient + update) of
 16 uneven structures run as one batch: eager, over several streams, and as
 a replayed hipGraph.
+
+A fourth, "large_cell", times evaluations of the 64-atom Si cell at the
+Gamma point (128^3 box, 128 bands) with the FP64 MFMA rate of its QR
+passes.
 """
 import argparse
 import json
@@ -254,6 +260,126 @@ def batch_bench(steps, warmup, streams=4):
   return out
 
 
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def qr_flops(cls, plan):
+  """Algorithmic real flops of one evaluation's launches of a QR class
+  (complex multiply-add = 8 flops; triangular factors count their nonzero
+  half): the Gram pass computes the nb(nb+1)/2 entries of a Hermitian
+  matrix, the forward apply multiplies by an upper-triangular R^-1, the
+  backward Gram is a full nb x nb product and the backward apply a
+  triangular plus a full one."""
+  P = plan.ns * plan.nk
+  ng, nb = plan.num_g, plan.nb
+  per = {"qr_gram": nb * (nb + 1) / 2, "qr_apply": nb * (nb + 1) / 2,
+         "qr_bwd_gram": nb * nb, "qr_bwd_apply": nb * (nb + 1) / 2 + nb * nb}
+  if cls not in per:
+    return None
+  return 8.0 * P * ng * per[cls]
+
+
+def large_cell_bench(steps, warmup):
+  """Energy + gradient evaluations of the 64-atom Si cell at the Gamma
+  point (128^3 box, 128 bands, 48 Ha, 137059 coefficients per band), with
+  per-class durations, the HBM rate of the FFT passes and the FP64 MFMA
+  rate of the QR passes."""
+  import numpy as np
+  import torch
+  from jrystal_amd import workloads
+  from jrystal_amd.objective import EnergyPlan
+  wl = workloads.si64()
+  p = wl.params(0)
+  plan = EnergyPlan.from_workload(wl)
+  wr = torch.tensor(p["w_re"], device=plan.device)
+  wi = torch.tensor(p["w_im"], device=plan.device)
+  out = {}
+  print(f"bench: large cell {wl.name}", file=sys.stderr, flush=True)
+  for _ in range(warmup):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  torch.cuda.synchronize()
+  dt = (time.perf_counter() - t0) / steps
+  e = out["energies"].tolist()
+  if not np.all(np.isfinite(e)):
+    raise RuntimeError(f"large cell: non-finite energies {e}")
+  plan.set_timing(True)
+  nprof = 3
+  for _ in range(nprof):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  timing = plan.read_timing()
+  plan.set_timing(False)
+  if plan.row_path:
+    timing = {ROW_PATH_CLASSES.get(k, k): v for k, v in timing.items()}
+  res = {"workload": wl.name, "atoms": int(wl.crystal.num_atom),
+         "grid": list(wl.grid_sizes), "num_bands": int(plan.nb),
+         "num_g": int(plan.num_g), "steps": steps,
+         "ms_per_eval": dt * 1e3, "evals_per_s": 1.0 / dt,
+         "energy_total_Ha": e[4],
+         "kernel_ms_per_eval": {k: round(v[0] / nprof, 4)
+                                for k, v in timing.items()},
+         "fft_hbm": {}, "qr_fp64_mfma": {}}
+  chunks = plan.ns * max(1, -(-plan.nk * plan.nb // plan.chunk_states))
+  for k, (ms, n) in timing.items():
+    sec = ms / nprof * 1e-3
+    b = algorithmic_bytes(k, plan, chunks)
+    f = qr_flops(k, plan)
+    if f is not None:
+      tf = f / sec / 1e12
+      res["qr_fp64_mfma"][k] = {"alg_tflops": tf,
+                                "frac_of_peak": tf / FP64_MFMA_PEAK_TFLOPS}
+    elif b is not None:
+      gbs = b / sec / 1e9
+      res["fft_hbm"][k] = {"alg_gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
+  res["fp64_mfma_peak_tflops"] = FP64_MFMA_PEAK_TFLOPS
+  res["mfma_counters"] = measured_mfma()
+  plan.close()
+  del plan, out, wr, wi
+  torch.cuda.empty_cache()
+  return res
+
+
+MFMA_PROFILE = os.path.join(
+  os.path.dirname(os.path.abspath(__file__)), "profiles",
+  "qr_si64-128^3-gamma-128b_pmc-mfma.md")
+
+
+def measured_mfma():
+  """Per-kernel FP64 MFMA figures of the large cell from the committed
+  rocprofv3 counter summary (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 flops over
+  the kernel's traced duration), or None."""
+  if not os.path.exists(MFMA_PROFILE):
+    return None
+  text = open(MFMA_PROFILE).read()
+  start = text.find("```json")
+  if start < 0:
+    return None
+  end = text.find("```", start + 7)
+  return json.loads(text[start + 7:end])
+
+
+def large_cell_probe():
+  """Workload of the large-cell rocprofv3 runs: two energy + gradient
+  evaluations of the 64-atom Gamma cell."""
+  import torch
+  from jrystal_amd import workloads
+  from jrystal_amd.objective import EnergyPlan
+  wl = workloads.si64()
+  p = wl.params(0)
+  plan = EnergyPlan.from_workload(wl)
+  wr = torch.tensor(p["w_re"], device="cuda")
+  wi = torch.tensor(p["w_im"], device="cuda")
+  out = {}
+  for _ in range(2):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  torch.cuda.synchronize()
+  print(f"large cell probe done: E = {float(out['energies'][4]):.10f}",
+        flush=True)
+
+
 TRAFFIC_PROFILE = os.path.join(
   os.path.dirname(os.path.abspath(__file__)), "profiles",
   "energy_grad_si8-cube-64k-32b_pmc-traffic.md")
@@ -333,9 +459,16 @@ def main():
   ap.add_argument("--batch-steps", type=int, default=5)
   ap.add_argument("--traffic-probe", action="store_true",
                   help="run the rocprofv3 counter workload and exit")
+  ap.add_argument("--no-large-cell", action="store_true")
+  ap.add_argument("--large-cell-steps", type=int, default=5)
+  ap.add_argument("--large-cell-probe", action="store_true",
+                  help="run the large-cell rocprofv3 workload and exit")
   args = ap.parse_args()
   if args.traffic_probe:
     traffic_probe()
+    return
+  if args.large_cell_probe:
+    large_cell_probe()
     return
   if args.gpus != 1:
     print(f"bench.py: --gpus {args.gpus} is not supported; this path runs "
@@ -419,6 +552,10 @@ def main():
   if not args.no_batch:
     batch = batch_bench(args.batch_steps, 1)
 
+  large = None
+  if not args.no_large_cell:
+    large = large_cell_bench(args.large_cell_steps, 1)
+
   conv = None
   if not args.no_convergence:
     conv, conv_ctx = convergence(args.conv_seconds)
@@ -460,6 +597,7 @@ def main():
     "cpu_baseline": cpu,
     "convergence": conv,
     "batch": batch,
+    "large_cell": large,
   }
   print(json.dumps(line))
 

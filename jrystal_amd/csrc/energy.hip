@@ -30,43 +30,76 @@ int launch_rho_to_complex(const double* rho, double2* out, int ns, int64_t N,
   return JRX_OK;
 }
 
+// Cartesian G and |G|^2 of flat box index i (box axes nx, ny, nz of cp).
+__device__ inline double box_g2(const CellParams& cp, int64_t i, double* g) {
+  const int iz = (int)(i % cp.nz);
+  const int iy = (int)((i / cp.nz) % cp.ny);
+  const int ix = (int)(i / ((int64_t)cp.ny * cp.nz));
+  const int mx = freq(ix, cp.nx), my = freq(iy, cp.ny), mz = freq(iz, cp.nz);
+  for (int d = 0; d < 3; ++d)
+    g[d] = mx * cp.B[d] + my * cp.B[3 + d] + mz * cp.B[6 + d];
+  return g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+}
+
+// S(G) = sum_a Z_a e^{+i G.R_a}
+__device__ inline double2 structure_factor(const CellParams& cp,
+                                           const double* g) {
+  double sr = 0.0, si = 0.0;
+  for (int a = 0; a < cp.natom; ++a) {
+    const double ph =
+        g[0] * cp.pos[3 * a] + g[1] * cp.pos[3 * a + 1] + g[2] * cp.pos[3 * a + 2];
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    sr += cp.charge[a] * cs;
+    si += cp.charge[a] * sn;
+  }
+  return make_double2(sr, si);
+}
+
+// out[i] = S(G_i) / (|G_i|^2 + 1e-10), 0 at G = 0.
+__global__ __launch_bounds__(kRedBlock) void structure_term_kernel(
+    CellParams cp, double2* __restrict__ out) {
+  const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
+  const int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
+  if (i >= N) return;
+  double g[3];
+  const double g2 = box_g2(cp, i, g);
+  out[i] = i == 0 ? czero() : cscale(structure_factor(cp, g), 1.0 / (g2 + 1e-10));
+}
+
+int launch_structure_term(const CellParams& cp, double2* out, hipStream_t s) {
+  const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
+  hipLaunchKernelGGL(structure_term_kernel, dim3(ceil_div(N, kRedBlock)),
+                     dim3(kRedBlock), 0, s, cp, out);
+  JRX_LAUNCH_CHECK();
+  return JRX_OK;
+}
+
 __global__ __launch_bounds__(kRedBlock) void potential_rec_kernel(
     const double2* __restrict__ rho_rec, double2* __restrict__ vrec,
     double* __restrict__ eh_part, double* __restrict__ ee_part,
-    CellParams cp) {
+    CellParams cp, const double2* __restrict__ sterm) {
   __shared__ double red[kRedBlock];
   const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
   const int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
   double eh = 0.0, ee = 0.0;
   if (i < N) {
-    const int iz = (int)(i % cp.nz);
-    const int iy = (int)((i / cp.nz) % cp.ny);
-    const int ix = (int)(i / ((int64_t)cp.ny * cp.nz));
-    const int mx = freq(ix, cp.nx), my = freq(iy, cp.ny), mz = freq(iz, cp.nz);
-    const double gx = mx * cp.B[0] + my * cp.B[3] + mz * cp.B[6];
-    const double gy = mx * cp.B[1] + my * cp.B[4] + mz * cp.B[7];
-    const double gz = mx * cp.B[2] + my * cp.B[5] + mz * cp.B[8];
-    const double g2 = gx * gx + gy * gy + gz * gz;
+    double g[3];
+    const double g2 = box_g2(cp, i, g);
     double2 out = czero();
     if (i != 0) {
-      double sr = 0.0, si = 0.0;  // S(G) = sum_a Z_a e^{+i G.R_a}
-      for (int a = 0; a < cp.natom; ++a) {
-        const double ph = gx * cp.pos[3 * a] + gy * cp.pos[3 * a + 1] +
-                          gz * cp.pos[3 * a + 2];
-        double sn, cs;
-        sincos(ph, &sn, &cs);
-        sr += cp.charge[a] * cs;
-        si += cp.charge[a] * sn;
-      }
+      // t = S(G) / (|G|^2 + 1e-10), precomputed per plan or formed here
+      const double2 t =
+          sterm ? sterm[i]
+                : cscale(structure_factor(cp, g), 1.0 / (g2 + 1e-10));
       const double2 r = rho_rec[i];
       const double inv_g2 = 1.0 / g2;
-      const double inv_g2e = 1.0 / (g2 + 1e-10);
       const double dN = (double)N;
       eh = (2.0 * M_PI * cp.vol / (dN * dN)) * (r.x * r.x + r.y * r.y) * inv_g2;
-      ee = -(4.0 * M_PI / dN) * (sr * r.x - si * r.y) * inv_g2e;
+      ee = -(4.0 * M_PI / dN) * (t.x * r.x - t.y * r.y);
       const double ch = 4.0 * M_PI * cp.vol / dN * inv_g2;
-      const double ce = -4.0 * M_PI * inv_g2e;
-      out = make_double2(ch * r.x + ce * sr, ch * r.y - ce * si);
+      out = make_double2(ch * r.x - 4.0 * M_PI * t.x,
+                         ch * r.y + 4.0 * M_PI * t.y);
     }
     vrec[i] = out;
   }
@@ -80,11 +113,12 @@ __global__ __launch_bounds__(kRedBlock) void potential_rec_kernel(
 
 int launch_potential_rec(const double2* rho_rec, double2* vrec,
                          double* eh_part, double* ee_part,
-                         const CellParams& cp, int* nparts, hipStream_t s) {
+                         const CellParams& cp, int* nparts, hipStream_t s,
+                         const double2* sterm) {
   const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
   const int nb = ceil_div(N, kRedBlock);
   hipLaunchKernelGGL(potential_rec_kernel, dim3(nb), dim3(kRedBlock), 0, s,
-                     rho_rec, vrec, eh_part, ee_part, cp);
+                     rho_rec, vrec, eh_part, ee_part, cp, sterm);
   JRX_LAUNCH_CHECK();
   *nparts = nb;
   return JRX_OK;

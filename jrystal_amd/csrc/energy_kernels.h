@@ -25,9 +25,16 @@ int launch_rho_to_complex(const double* rho, double2* out, int ns, int64_t N,
 //   E_H   = (2 pi vol / N^2) sum_{G!=0} |rho_rec|^2 / |G|^2
 //   E_ext = -(4 pi / N) sum_{G!=0} Re(S rho_rec) / (|G|^2 + 1e-10).
 // vrec may alias rho_rec.  Returns the number of partials written.
+// sterm (optional): S(G) / (|G|^2 + 1e-10) per box point, as
+// launch_structure_term writes it; without it S(G) is formed per point.
 int launch_potential_rec(const double2* rho_rec, double2* vrec,
                          double* eh_part, double* ee_part,
-                         const CellParams& cp, int* nparts, hipStream_t s);
+                         const CellParams& cp, int* nparts, hipStream_t s,
+                         const double2* sterm = nullptr);
+
+// out[i] = S(G_i) / (|G_i|^2 + 1e-10) (0 at G = 0) over cp's box: the
+// geometry-only factor of the external term, formed once per plan.
+int launch_structure_term(const CellParams& cp, double2* out, hipStream_t s);
 
 // v_s(r) = Re(vbox(r)) + v_xc,s(r) and partials of E_xc (LDA exchange).
 int launch_potential_real(const double2* vbox, const double* rho, double* v,

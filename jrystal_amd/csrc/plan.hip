@@ -109,6 +109,7 @@ struct jrx_plan {
   short* d_g_m = nullptr;
   double* d_kpts = nullptr;
   double* d_occ = nullptr;
+  double2* d_sterm = nullptr;  // S(G) / (|G|^2 + 1e-10) on the rho box
   int chunk = 0;   // states per FFT chunk (never straddles spins)
   bool force_generic = false;  // run the runtime-radix kernels at 64^3
   bool keep_planes = false;    // plane buffer holds all states
@@ -387,6 +388,14 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
     p->cpz.B[i] = p->B[6 + i];
     p->cpz.B[6 + i] = p->B[i];
   }
+  // The geometry-only factor of the external term, S(G) / (|G|^2 + 1e-10),
+  // on the box rho lives in (it does not change between evaluations).
+  JRX_CHECK_HIP(hipMalloc(&p->d_sterm, (size_t)p->N * sizeof(double2)));
+  p->allocs.push_back(p->d_sterm);
+  if ((rc = launch_structure_term(p->rowpath ? p->cpz : p->cp, p->d_sterm,
+                                  nullptr)))
+    return rc;
+  JRX_CHECK_HIP(hipStreamSynchronize(nullptr));
   p->gsw = gram_split(p->P, ng, nb);
   p->gsq = p->gsw;
   const int64_t kin_gather =
@@ -1167,7 +1176,8 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   if (gb)  // density gradients of the GGA terms, from fftn(rho)
     JRX_T(JRX_K_POTENTIAL,
           xc_stage_gradients(box, gb, cpb, L0, p->Ly, L2, s));
-  JRX_T(JRX_K_POTENTIAL, launch_potential_rec(box, box, eh, ee, cpb, &nrec, s));
+  JRX_T(JRX_K_POTENTIAL,
+        launch_potential_rec(box, box, eh, ee, cpb, &nrec, s, p->d_sterm));
   JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, true,
                                1.0 / (double)p->N, L0, p->Ly, L2, s));
   if (p->xc == JRX_XC_LDA_X)

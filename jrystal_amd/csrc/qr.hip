@@ -21,7 +21,7 @@ GramSplit gram_split(int64_t P, int ng, int nb) {
   const int tiles = (ng + kQrRows - 1) / kQrRows;
   // about 2048 blocks in all; the wide kernels do four to sixteen times
   // the work per row, and each chunk adds an nb x nb partial to reduce
-  const int64_t target = nb > kNarrowBands ? 512 : 2048;
+  const int64_t target = nb > kNarrowBands ? 256 : 2048;
   int64_t want = (target + P - 1) / P;  // chunks per problem
   if (want < 1) want = 1;
   if (want > tiles) want = tiles;
@@ -203,7 +203,8 @@ int launch_gram_wide(const double* wr, const double* wi, const double2* qa,
   return JRX_OK;
 }
 
-constexpr int kCholWideBlock = 512;
+constexpr int kCholWideBlock = 1024;
+constexpr int kInvLanes = 8;  // threads per row in the inverse
 
 // Element (i, j), i <= j, of an upper triangle packed row by row.
 __device__ inline int up_index(int i, int j, int nb) {
@@ -211,21 +212,29 @@ __device__ inline int up_index(int i, int j, int nb) {
 }
 
 // chol_inv_kernel for nb > 64: the upper triangle, packed, is the only
-// matrix in LDS (132 KB at nb = 128); R is factored in place and then
-// inverted in place column by column,
+// matrix in LDS (132 KB at nb = 128), factored and inverted in place with
+// one barrier per step.
+// Factorisation, step j (right-looking, on the unscaled row j):
+//   A[r][c] -= conj(A[j][r]) A[j][c] / A[j][j]   (j < r <= c),
+// waves over rows r, lanes over columns c; row j itself is scaled to
+// R[j][c] = A[j][c] / sqrt(A[j][j]) during step j + 1, which does not read
+// it (the pivots are kept aside in piv).
+// Inverse, column j (rows are independent given R):
 //   X[j][j] = 1 / R[j][j],  X[i][j] = -sum_{k=i}^{j-1} X[i][k] R[k][j] / R[j][j],
-// with column j of R copied aside first.
+// kInvLanes threads per row i sharing the sum; column j of R is copied
+// aside (double-buffered) one step ahead, before X overwrites it.
 __global__ __launch_bounds__(kCholWideBlock) void chol_inv_wide_kernel(
     const double2* __restrict__ S, double2* __restrict__ rinv, int nb,
     int* __restrict__ flag_out, const int* __restrict__ flag_in) {
   extern __shared__ double2 sm[];
   const int np = nb * (nb + 1) / 2;
   double2* A = sm;
-  double2* col = sm + np;
-  double* red = reinterpret_cast<double*>(col + nb);
+  double2* col = sm + np;  // [2][nb]
+  double* piv = reinterpret_cast<double*>(col + 2 * nb);  // [nb]
+  double* red = piv + nb;                                  // [B / 64]
   const int64_t p = blockIdx.x;
-  const int tid = threadIdx.x;
-  constexpr int B = kCholWideBlock;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int B = kCholWideBlock, NW = B / 64;
   double2* out = rinv + p * nb * nb;
   if (flag_in && flag_in[p]) {  // uniform over the block
     for (int e = tid; e < nb * nb; e += B) {
@@ -253,41 +262,88 @@ __global__ __launch_bounds__(kCholWideBlock) void chol_inv_wide_kernel(
     const double d2 = block_sum(a, red);
     if (tid == 0) flag_out[p] = (d2 < kQrOnePass * kQrOnePass) ? 1 : 0;
   }
-  for (int j = 0; j < nb; ++j) {
-    const int jj = up_index(j, j, nb);
-    if (tid == 0) {
-      const double d = A[jj].x;
-      // NaN for a non-positive pivot: sqrt of a negative number.
-      A[jj] = make_double2(d > 0.0 ? sqrt(d) : sqrt(-1.0 - d * 0.0), 0.0);
+  // NaN for a non-positive pivot: sqrt of a negative number.
+  auto root = [](double d) { return d > 0.0 ? sqrt(d) : sqrt(-1.0 - d * 0.0); };
+  for (int j = 0; j <= nb; ++j) {
+    if (j >= 1) {  // scale row j - 1 (its pivot is piv[j - 1])
+      const int jm = j - 1, base = up_index(jm, jm, nb);
+      const double sq = root(piv[jm]);
+      for (int c = tid; c < nb - jm; c += B)
+        A[base + c] = c == 0 ? make_double2(sq, 0.0) : cscale(A[base + c], 1.0 / sq);
     }
-    __syncthreads();
-    const double d = A[jj].x;
-    for (int c = j + 1 + tid; c < nb; c += B) A[jj + c - j] = cscale(A[jj + c - j], 1.0 / d);
-    __syncthreads();
-    const int m = nb - j - 1;
-    for (int e = tid; e < m * m; e += B) {
-      const int r = j + 1 + e / m, c = j + 1 + e % m;
-      if (c >= r)
-        A[up_index(r, c, nb)] =
-            csub(A[up_index(r, c, nb)], cmulc(A[jj + r - j], A[jj + c - j]));
+    if (j < nb) {
+      const int base = up_index(j, j, nb);
+      const double dd = A[base].x;
+      if (tid == 0) piv[j] = dd;
+      const double inv = 1.0 / dd;
+      // two rows per pass, every load issued before the stores (nb <= 128:
+      // columns r + lane and r + lane + 64 cover a row)
+      for (int r = j + 1 + wave; r < nb; r += 2 * NW) {
+        const int r2 = r + NW;
+        const bool two = r2 < nb;
+        double2 x[2][2], y[2][2], f[2];
+        int rb[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int rr = h ? (two ? r2 : r) : r;
+          const double2 ajr = A[base + rr - j];
+          f[h] = make_double2(ajr.x * inv, -ajr.y * inv);  // conj / dd
+          rb[h] = up_index(rr, rr, nb);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int c = rr + lane + 64 * u;
+            x[h][u] = c < nb ? A[rb[h] + c - rr] : czero();
+            y[h][u] = c < nb ? A[base + c - j] : czero();
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (h && !two) break;
+          const int rr = h ? r2 : r;
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int c = rr + lane + 64 * u;
+            if (c < nb) A[rb[h] + c - rr] = csub(x[h][u], cmul(f[h], y[h][u]));
+          }
+        }
+      }
     }
     __syncthreads();
   }
+  // inverse, in place
+  const int i = tid / kInvLanes, t = tid % kInvLanes;
+  const int ib = i < nb ? up_index(i, i, nb) : 0;
+  if (tid <= 0) col[0] = A[0];  // column 0 of R
+  __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    for (int k = tid; k <= j; k += B) col[k] = A[up_index(k, j, nb)];
-    __syncthreads();
-    const double rjj = col[j].x;
-    for (int i = tid; i < j; i += B) {
-      double2 acc = czero();
-      for (int k = i; k < j; ++k) acc = cadd(acc, cmul(A[up_index(i, k, nb)], col[k]));
-      A[up_index(i, j, nb)] = cscale(acc, -1.0 / rjj);
+    const double2* cj = col + (j & 1) * nb;
+    if (i < j) {
+      double2 acc = czero(), acc2 = czero();
+      int k = i + t;
+      for (; k + kInvLanes < j; k += 2 * kInvLanes) {
+        acc = cadd(acc, cmul(A[ib + k - i], cj[k]));
+        acc2 = cadd(acc2, cmul(A[ib + k + kInvLanes - i], cj[k + kInvLanes]));
+      }
+      if (k < j) acc = cadd(acc, cmul(A[ib + k - i], cj[k]));
+      acc = cadd(acc, acc2);
+#pragma unroll
+      for (int m = kInvLanes / 2; m > 0; m >>= 1) {
+        acc.x += __shfl_xor(acc.x, m, kInvLanes);
+        acc.y += __shfl_xor(acc.y, m, kInvLanes);
+      }
+      if (t == 0) A[ib + j - i] = cscale(acc, -1.0 / cj[j].x);
+    } else if (i == j && t == 0) {
+      A[ib] = make_double2(1.0 / cj[j].x, 0.0);
     }
-    if (tid == 0) A[up_index(j, j, nb)] = make_double2(1.0 / rjj, 0.0);
+    if (j + 1 < nb) {  // column j + 1 of R, untouched until the next step
+      double2* cn = col + ((j + 1) & 1) * nb;
+      for (int k = tid; k <= j + 1; k += B) cn[k] = A[up_index(k, j + 1, nb)];
+    }
     __syncthreads();
   }
   for (int e = tid; e < nb * nb; e += B) {
-    const int i = e / nb, j = e - i * nb;
-    out[e] = (j >= i) ? A[up_index(i, j, nb)] : czero();
+    const int r = e / nb, c = e - r * nb;
+    out[e] = (c >= r) ? A[up_index(r, c, nb)] : czero();
   }
 }
 
@@ -688,21 +744,34 @@ int launch_gram_bmajor(const double2* q, const double2* qb, double2* partial,
   return JRX_OK;
 }
 
+// A block sums kRedOut consecutive outputs: its kRedGroups thread groups
+// take chunks c = g, g + kRedGroups, ... in order, then the group sums are
+// added in group order (a fixed order for every launch).
+constexpr int kRedOut = 32, kRedGroups = kQrBlock / kRedOut;
+
 __global__ __launch_bounds__(kQrBlock) void reduce_partials_kernel(
     const double2* __restrict__ partial, double2* __restrict__ out,
     int nchunk, int nout) {
+  __shared__ double2 acc[kRedGroups][kRedOut];
   const int64_t p = blockIdx.y;
-  const int o = blockIdx.x * kQrBlock + threadIdx.x;
-  if (o >= nout) return;
+  const int ol = threadIdx.x % kRedOut, grp = threadIdx.x / kRedOut;
+  const int o = blockIdx.x * kRedOut + ol;
   double2 a = czero();
-  for (int c = 0; c < nchunk; ++c) a = cadd(a, partial[(p * nchunk + c) * nout + o]);
-  out[p * nout + o] = a;
+  if (o < nout)
+    for (int c = grp; c < nchunk; c += kRedGroups)
+      a = cadd(a, partial[(p * nchunk + c) * nout + o]);
+  acc[grp][ol] = a;
+  __syncthreads();
+  if (grp == 0 && o < nout) {
+    for (int g = 1; g < kRedGroups; ++g) a = cadd(a, acc[g][ol]);
+    out[p * nout + o] = a;
+  }
 }
 
 int launch_reduce_partials(const double2* partial, double2* out, int64_t P,
                            int nchunk, int nb, hipStream_t s) {
   const int nout = nb * nb;
-  dim3 grid(ceil_div(nout, kQrBlock), (unsigned)P), block(kQrBlock);
+  dim3 grid(ceil_div(nout, kRedOut), (unsigned)P), block(kQrBlock);
   hipLaunchKernelGGL(reduce_partials_kernel, grid, block, 0, s, partial, out,
                      nchunk, nout);
   JRX_LAUNCH_CHECK();
@@ -784,8 +853,8 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
 int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
                     hipStream_t s, int* flag_out, const int* flag_in) {
   if (nb > kNarrowBands) {
-    const size_t shm = ((size_t)nb * (nb + 1) / 2 + nb) * sizeof(double2) +
-                       (kCholWideBlock / 64) * sizeof(double);
+    const size_t shm = ((size_t)nb * (nb + 1) / 2 + 2 * nb) * sizeof(double2) +
+                       (nb + kCholWideBlock / 64) * sizeof(double);
     hipLaunchKernelGGL(chol_inv_wide_kernel, dim3((unsigned)P),
                        dim3(kCholWideBlock), shm, s, S, rinv, nb, flag_out,
                        flag_in);
@@ -1029,8 +1098,8 @@ size_t qr_lds_bytes(int nb) {
   const size_t gram = 2 * (size_t)kWideSlab * (nbp + 1) * z;
   const size_t fwd = (size_t)kWideK * nbp * z;
   const size_t bwd = 2 * (size_t)kWideKB * nbp * z;
-  const size_t chol = ((size_t)nb * (nb + 1) / 2 + nb) * z +
-                      (kCholWideBlock / 64) * sizeof(double);
+  const size_t chol = ((size_t)nb * (nb + 1) / 2 + 2 * nb) * z +
+                      (nb + kCholWideBlock / 64) * sizeof(double);
   return std::max(std::max(gram, fwd), std::max(bwd, chol));
 }
 
