@@ -502,12 +502,15 @@ zplane_vpsi_kernel(
 #pragma unroll
         for (int m = 0; m < V; ++m) v[m] = sm[Z::at(j + 8 * m, xa)];
         Z::template fft<false>(v, typename Z::ColAddr{sm, xa}, j, stw, rtw);
-        // off-mask rows go to the slice's pad element (index nrows)
+        // off-mask rows go to the slice's pad element (index nrows).  Plain
+        // stores: a column's rows start anywhere in a cache line, and the
+        // L2 merges the partial lines that neighbouring columns complete
+        // (nontemporal stores wrote them back separately: +50% writes).
 #pragma unroll
         for (int t = 0; t < V; ++t) {
           const int r = ri[p][t];
-          if (JRX_ZBRANCHFREE) nt_store(buf + (unsigned)(r < 0 ? geo.nrows : r), v[t]);
-          else if (r >= 0) nt_store(buf + r, v[t]);
+          if (JRX_ZBRANCHFREE) buf[(unsigned)(r < 0 ? geo.nrows : r)] = v[t];
+          else if (r >= 0) buf[r] = v[t];
         }
       }
     }
