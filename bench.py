@@ -398,6 +398,27 @@ ROW_PATH_CLASSES = {"plane_ifft": "row_ifft",
                     "plane_fft_gather": "row_fft_gather"}
 
 
+SQ_PROFILE = os.path.join(
+  os.path.dirname(os.path.abspath(__file__)), "profiles",
+  "energy_grad_si8-cube-64k-32b_pmc-sq.md")
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def measured_sq(cls):
+  """Per-launch SQ figures of a kernel class from the committed rocprofv3
+  SQ counter summary (valu_f64_flop = 64 lanes x (ADD + MUL + 2 FMA) F64
+  instructions, valu_instr = VALU instructions), or None."""
+  name = KERNEL_NAMES.get(cls)
+  if name is None or not os.path.exists(SQ_PROFILE):
+    return None
+  text = open(SQ_PROFILE).read()
+  start = text.find("```json")
+  if start < 0:
+    return None
+  end = text.find("```", start + 7)
+  return json.loads(text[start + 7:end]).get(name)
+
+
 def measured_traffic(cls):
   """HBM bytes per launch of a kernel class from the committed rocprofv3
   FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
@@ -535,6 +556,19 @@ def main():
                 "traffic": measured_traffic(dom),
                 "avg_launch_ms": per_launch_ms,
                 "algorithmic_bytes_per_launch": per_launch_bytes}
+    sq = measured_sq(dom)
+    if sq and sq.get("valu_f64_flop"):
+      # the FFT passes' other bound: the FP64 VALU (counted flops of one
+      # launch over the live launch duration; an FP64 add occupies an issue
+      # slot like an FMA, so the issue fraction is given as well: VALU
+      # instructions x 4 cycles per wave64 over 1024 SIMDs x 2.4 GHz)
+      t = per_launch_ms * 1e-3
+      tf = sq["valu_f64_flop"] / t / 1e12
+      roofline["fp64_valu"] = {
+        "achieved": tf, "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": tf / FP64_VALU_PEAK_TFLOPS,
+        "flop_per_launch": sq["valu_f64_flop"],
+        "issue_frac": sq.get("valu_instr", 0) * 4 / (1024 * 2.4e9 * t)}
   breakdown = {k: round(v[0] / nprof, 4) for k, v in timing.items()}
   config = {"workload": wl.name, "atoms": int(wl.crystal.num_atom),
             "Z": 14, "grid": list(wl.grid_sizes), "k_mesh": list(wl.k_grid),

@@ -330,6 +330,19 @@ int jrx_external(const double* rho_rec, const double* gvec,
 int jrx_xc_lda(const double* rho, int ns, int64_t N, double vol, double* out,
                double* grad, const double* grad_out, void* workspace,
                size_t workspace_bytes, void* stream);
+/* energy.xc_energy (energy.py:185-211) for the '+'-joined xc strings of
+ * xc.py:222-253 (xc_flags: an OR of JRX_XC_* terms) on an unpolarised
+ * density rho [nx][ny][nz] with the G grid gvec [nx][ny][nz][3]:
+ *   out[0] = vol/N sum_r eps(rho, sigma) rho,
+ *   sigma = sum_d |ifftn(i G_d fftn(rho))|^2 (complex gradients, as
+ *   sigma_r_fn of xc.py:88-106 forms them),
+ * and, when grad != NULL, grad = grad_out[0] * dE_xc/drho (the adjoint of
+ * that discretisation).  Workspace: jrx_xc_workspace_bytes, 256-B aligned. */
+size_t jrx_xc_workspace_bytes(int nx, int ny, int nz, int xc_flags);
+int jrx_xc(const double* rho, const double* gvec, int nx, int ny, int nz,
+           double vol, int xc_flags, double* out, double* grad,
+           const double* grad_out, void* workspace, size_t workspace_bytes,
+           void* stream);
 /* Element-wise helpers of the autograd glue: y = (x, 0); y = Re x (n
  * complex); y = alpha[0] * x (alpha a device scalar). */
 int jrx_real_to_complex(const double* x, double* y, int64_t n, void* stream);

@@ -100,6 +100,8 @@ SIGNATURES = {
   "jrx_hartree": [_P, _P, _I, _I64, _D, _I, _P, _P, _P, _P, _SZ, _P],
   "jrx_external": [_P, _P, _P, _P, _I, _I, _I64, _P, _P, _P, _P, _SZ, _P],
   "jrx_xc_lda": [_P, _I, _I64, _D, _P, _P, _P, _P, _SZ, _P],
+  "jrx_xc_workspace_bytes": [_I, _I, _I, _I],
+  "jrx_xc": [_P, _P, _I, _I, _I, _D, _I, _P, _P, _P, _P, _SZ, _P],
   "jrx_adam_update": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _D, _I64, _P],
   "jrx_real_to_complex": [_P, _P, _I64, _P],
   "jrx_complex_real": [_P, _P, _I64, _P],
@@ -144,6 +146,7 @@ KERNEL_CLASSES = (
 _RESTYPES = {"jrx_last_error": ctypes.c_char_p,
              "jrx_version": ctypes.c_char_p,
              "jrx_qr_workspace_bytes": ctypes.c_size_t,
+             "jrx_xc_workspace_bytes": ctypes.c_size_t,
              "jrx_kinetic_workspace_bytes": ctypes.c_size_t,
              "jrx_reduce_workspace_bytes": ctypes.c_size_t,
              "jrx_ewald_workspace_bytes": ctypes.c_size_t,

@@ -309,6 +309,37 @@ class ExchangeLDA(torch.autograd.Function):
     return grad, None
 
 
+class ExchangeCorrelation(torch.autograd.Function):
+  """E_xc of the JRX_XC_* terms in `flags` on an unpolarised density
+  (1, nx, ny, nz) with the G grid (nx, ny, nz, 3): jrx_xc forward and
+  backward."""
+
+  @staticmethod
+  def forward(ctx, rho, gvec, vol, flags):
+    rho = rho.contiguous()
+    gvec = gvec.contiguous()
+    nx, ny, nz = rho.shape[1:]
+    out = torch.empty((), dtype=torch.float64, device=rho.device)
+    ws, wp = workspace(_L().jrx_xc_workspace_bytes(nx, ny, nz, flags))
+    _lib.check(_L().jrx_xc(_p(rho), _p(gvec), nx, ny, nz, _D(vol), flags,
+                           _p(out), None, None, wp, ws.numel() - 256, _s()))
+    ctx.save_for_backward(rho, gvec)
+    ctx.vol, ctx.flags = vol, flags
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    rho, gvec = ctx.saved_tensors
+    nx, ny, nz = rho.shape[1:]
+    grad = torch.empty_like(rho)
+    scratch = torch.empty((), dtype=torch.float64, device=rho.device)
+    ws, wp = workspace(_L().jrx_xc_workspace_bytes(nx, ny, nz, ctx.flags))
+    _lib.check(_L().jrx_xc(_p(rho), _p(gvec), nx, ny, nz, _D(ctx.vol),
+                           ctx.flags, _p(scratch), _p(grad),
+                           _p(g.contiguous()), wp, ws.numel() - 256, _s()))
+    return grad, None, None, None
+
+
 def ewald(positions, charges, gvec, vol, eta, translations):
   pos = as_device(positions, torch.float64).reshape(-1, 3)
   chg = as_device(charges, torch.float64).reshape(-1)

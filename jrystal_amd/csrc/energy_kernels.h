@@ -65,15 +65,21 @@ int launch_fill(double* p, double v, int64_t n, hipStream_t s);
 struct FftLen;
 bool xc_is_gga(int flags);
 // gb[3][N] = ifftn(i G_d rho_rec) (the density gradient boxes; cp's box)
+// (gvec, optional: the G grid [N][3] to use instead of cp's basis)
 int xc_stage_gradients(const double2* rho_rec, double2* gb,
                        const CellParams& cp, const FftLen& L0,
-                       const FftLen& L1, const FftLen& L2, hipStream_t s);
+                       const FftLen& L1, const FftLen& L2, hipStream_t s,
+                       const double* gvec = nullptr);
 // v = Re(vbox) (vbox may be NULL) + dE_xc/drho and partials of E_xc for
 // the JRX_XC_* terms in flags; for GGA terms gb holds the gradient boxes on
 // entry and is used as scratch.
 int xc_stage_potential(const double2* vbox, const double* rho, double* v,
                        double2* gb, double* exc_part, int* nparts,
                        const CellParams& cp, int flags, const FftLen& L0,
-                       const FftLen& L1, const FftLen& L2, hipStream_t s);
+                       const FftLen& L1, const FftLen& L2, hipStream_t s,
+                       const double* gvec = nullptr);
+
+// Twiddle tables of the plan-less FFT entry points (a per-device cache).
+int cached_fftlen(int n, FftLen* L);
 
 }  // namespace jrx

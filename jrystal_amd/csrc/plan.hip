@@ -34,7 +34,7 @@ static std::vector<double2> host_twiddles(int n) {
 static std::mutex g_tw_mutex;
 static std::map<std::pair<int, int>, double2*> g_tw_cache;
 
-static int cached_fftlen(int n, FftLen* L) {
+int cached_fftlen(int n, FftLen* L) {
   if (n < 1 || !fft_factorize(n, L)) {
     set_error("FFT size " + std::to_string(n) +
               " is not a product of 2, 3, 5 and 7");

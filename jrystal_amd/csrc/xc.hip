@@ -117,14 +117,25 @@ __device__ inline void box_g(const CellParams& cp, int64_t i, double* g) {
     g[d] = mx * cp.B[d] + my * cp.B[3 + d] + mz * cp.B[6 + d];
 }
 
+// G of box point i: from the caller's G grid gvec [N][3] when given, else
+// from cp's reciprocal basis.
+__device__ inline void point_g(const CellParams& cp, const double* gvec,
+                               int64_t i, double* g) {
+  if (gvec) {
+    for (int d = 0; d < 3; ++d) g[d] = gvec[3 * i + d];
+  } else {
+    box_g(cp, i, g);
+  }
+}
+
 // gb[d][i] = i G_d rho_rec[i]  (d = 0, 1, 2)
 __global__ void xc_grad_kernel(const double2* __restrict__ rho_rec,
                                double2* __restrict__ gb, CellParams cp,
-                               int64_t N) {
+                               const double* __restrict__ gvec, int64_t N) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double g[3];
-  box_g(cp, i, g);
+  point_g(cp, gvec, i, g);
   const double2 r = rho_rec[i];
   for (int d = 0; d < 3; ++d) gb[d * N + i] = make_double2(-g[d] * r.y, g[d] * r.x);
 }
@@ -159,11 +170,11 @@ __global__ __launch_bounds__(kRedBlock) void xc_point_kernel(
 
 // out[i] = sum_d (-i G_d) gb[d][i]  (out may alias gb[0])
 __global__ void xc_div_kernel(const double2* gb, double2* out, CellParams cp,
-                              int64_t N) {
+                              const double* __restrict__ gvec, int64_t N) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double g[3];
-  box_g(cp, i, g);
+  point_g(cp, gvec, i, g);
   double2 acc = czero();
   for (int d = 0; d < 3; ++d) {
     const double2 a = gb[d * N + i];
@@ -179,6 +190,23 @@ __global__ void xc_add_kernel(double* __restrict__ v,
   if (i < N) v[i] += 2.0 * y[i].x;
 }
 
+// out[0] = sum_j part[j] (one block, fixed order).
+__global__ __launch_bounds__(kRedBlock) void xc_sum_kernel(
+    const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[kRedBlock];
+  double a = 0.0;
+  for (int j = threadIdx.x; j < n; j += kRedBlock) a += part[j];
+  const double t = block_sum(a, red);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+// v[i] *= s[0]
+__global__ void xc_scale_kernel(double* __restrict__ v,
+                                const double* __restrict__ s, int64_t N) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) v[i] *= s[0];
+}
+
 }  // namespace
 
 bool xc_is_gga(int flags) {
@@ -187,10 +215,11 @@ bool xc_is_gga(int flags) {
 
 int xc_stage_gradients(const double2* rho_rec, double2* gb,
                        const CellParams& cp, const FftLen& L0,
-                       const FftLen& L1, const FftLen& L2, hipStream_t s) {
+                       const FftLen& L1, const FftLen& L2, hipStream_t s,
+                       const double* gvec) {
   const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
   hipLaunchKernelGGL(xc_grad_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s,
-                     rho_rec, gb, cp, N);
+                     rho_rec, gb, cp, gvec, N);
   JRX_LAUNCH_CHECK();
   return fft3d(gb, gb, 3, cp.nx, cp.ny, cp.nz, true, 1.0 / (double)N, L0, L1,
                L2, s);
@@ -199,7 +228,8 @@ int xc_stage_gradients(const double2* rho_rec, double2* gb,
 int xc_stage_potential(const double2* vbox, const double* rho, double* v,
                        double2* gb, double* exc_part, int* nparts,
                        const CellParams& cp, int flags, const FftLen& L0,
-                       const FftLen& L1, const FftLen& L2, hipStream_t s) {
+                       const FftLen& L1, const FftLen& L2, hipStream_t s,
+                       const double* gvec) {
   const int64_t N = (int64_t)cp.nx * cp.ny * cp.nz;
   const int nb = ceil_div(N, kRedBlock);
   const bool gga = xc_is_gga(flags);
@@ -212,7 +242,7 @@ int xc_stage_potential(const double2* vbox, const double* rho, double* v,
   int rc = fft3d(gb, gb, 3, cp.nx, cp.ny, cp.nz, false, 1.0, L0, L1, L2, s);
   if (rc) return rc;
   hipLaunchKernelGGL(xc_div_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s,
-                     gb, gb, cp, N);
+                     gb, gb, cp, gvec, N);
   JRX_LAUNCH_CHECK();
   rc = fft3d(gb, gb, 1, cp.nx, cp.ny, cp.nz, true, 1.0 / (double)N, L0, L1,
              L2, s);
@@ -224,3 +254,88 @@ int xc_stage_potential(const double2* vbox, const double* rho, double* v,
 }
 
 }  // namespace jrx
+
+using namespace jrx;
+
+namespace {
+// workspace of jrx_xc: fftn(rho) box, three gradient boxes (GGA), the
+// potential when the caller wants no gradient, and the energy partials
+struct XcWs {
+  size_t box, gb, v, part, total;
+};
+XcWs xc_ws_layout(int64_t N, int flags) {
+  XcWs w;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off = (off + b + 255) & ~(size_t)255;
+    return o;
+  };
+  w.box = take((size_t)N * sizeof(double2));
+  w.gb = take(xc_is_gga(flags) ? 3 * (size_t)N * sizeof(double2) : 0);
+  w.v = take((size_t)N * sizeof(double));
+  w.part = take((size_t)ceil_div(N, kRedBlock) * sizeof(double));
+  w.total = off;
+  return w;
+}
+}  // namespace
+
+extern "C" {
+
+size_t jrx_xc_workspace_bytes(int nx, int ny, int nz, int xc_flags) {
+  return xc_ws_layout((int64_t)nx * ny * nz, xc_flags).total;
+}
+
+int jrx_xc(const double* rho, const double* gvec, int nx, int ny, int nz,
+           double vol, int xc_flags, double* out, double* grad,
+           const double* grad_out, void* workspace, size_t workspace_bytes,
+           void* stream) {
+  JRX_REQUIRE(rho && gvec && out && workspace && nx >= 1 && ny >= 1 &&
+                  nz >= 1 && vol > 0.0,
+              "invalid xc arguments");
+  JRX_REQUIRE(!grad || grad_out, "grad requires grad_out");
+  JRX_REQUIRE(xc_flags >= 1 && xc_flags <= 15,
+              "xc_flags must be a nonzero OR of JRX_XC_* terms");
+  const int64_t N = (int64_t)nx * ny * nz;
+  const XcWs w = xc_ws_layout(N, xc_flags);
+  JRX_REQUIRE(workspace_bytes >= w.total && ((uintptr_t)workspace & 255) == 0,
+              "xc workspace too small or misaligned");
+  FftLen Lx, Ly, Lz;
+  int rc;
+  if ((rc = cached_fftlen(nx, &Lx)) || (rc = cached_fftlen(ny, &Ly)) ||
+      (rc = cached_fftlen(nz, &Lz)))
+    return rc;
+  char* ws = static_cast<char*>(workspace);
+  double2* box = reinterpret_cast<double2*>(ws + w.box);
+  double2* gb = xc_is_gga(xc_flags) ? reinterpret_cast<double2*>(ws + w.gb)
+                                    : nullptr;
+  double* v = grad ? grad : reinterpret_cast<double*>(ws + w.v);
+  double* part = reinterpret_cast<double*>(ws + w.part);
+  hipStream_t s = (hipStream_t)stream;
+  CellParams cp{};
+  cp.nx = nx;
+  cp.ny = ny;
+  cp.nz = nz;
+  cp.vol = vol;
+  if (gb) {
+    if ((rc = launch_rho_to_complex(rho, box, 1, N, s)) ||
+        (rc = fft3d(box, box, 1, nx, ny, nz, false, 1.0, Lx, Ly, Lz, s)) ||
+        (rc = xc_stage_gradients(box, gb, cp, Lx, Ly, Lz, s, gvec)))
+      return rc;
+  }
+  int nparts = 0;
+  if ((rc = xc_stage_potential(nullptr, rho, v, gb, part, &nparts, cp,
+                               xc_flags, Lx, Ly, Lz, s, gvec)))
+    return rc;
+  hipLaunchKernelGGL(xc_sum_kernel, dim3(1), dim3(kRedBlock), 0, s, part,
+                     nparts, out);
+  JRX_LAUNCH_CHECK();
+  if (grad) {
+    hipLaunchKernelGGL(xc_scale_kernel, dim3(ceil_div(N, 256)), dim3(256), 0,
+                       s, grad, grad_out, N);
+    JRX_LAUNCH_CHECK();
+  }
+  return JRX_OK;
+}
+
+}  // extern "C"

@@ -45,14 +45,22 @@ def kinetic(g_vector_grid, kpts, coeff_grid, occupation=None):
 
 def xc_energy(density_grid, g_vector_grid, vol, xc_type="lda_x",
               kohn_sham=False):
-  """energy.py:185-211 for LDA exchange (xc.py:54-64)."""
+  """energy.py:185-211: LDA exchange with or without spin (xc.py:54-64),
+  and for unpolarised densities any '+'-joined sum of lda_x, lda_c_pw,
+  gga_x_pbe and gga_c_pbe (xc.py:67-106,222-253; the gradient boxes are
+  formed as sigma_r_fn forms them)."""
   assert density_grid.ndim == 4, "density_grid must contains spin axis"
-  if xc_type != "lda_x":
-    raise NotImplementedError(
-      f"xc {xc_type!r}: only lda_x is on this path (GGA is a next step)")
   if kohn_sham:
     raise NotImplementedError("kohn_sham xc potentials are not on this path")
-  return _ops.ExchangeLDA.apply(density_grid, float(vol))
+  if xc_type == "lda_x":
+    return _ops.ExchangeLDA.apply(density_grid, float(vol))
+  from .objective import xc_terms
+  flags = xc_terms(xc_type)
+  if density_grid.shape[0] != 1:
+    raise NotImplementedError(
+      f"xc {xc_type!r}: spin-polarised densities support lda_x only")
+  gvec = _ops.as_device(g_vector_grid, torch.float64)
+  return _ops.ExchangeCorrelation.apply(density_grid, gvec, float(vol), flags)
 
 
 def ewald_coulomb_repulsion(positions, charges, g_vector_grid, vol,

@@ -87,3 +87,37 @@ def test_cli_with_reference_default_xc(tmp_path):
   h = out.total_energy_history
   assert len(h) == 40 and h[-1] < h[0]
   assert np.isfinite(out.energies["xc"]) and out.energies["xc"] < 0
+
+
+@pytest.mark.parametrize("xc", ["lda_c_pw", "lda_x+lda_c_pw", "gga_x_pbe",
+                                "gga_x_pbe+gga_c_pbe"])
+def test_public_xc_energy_and_gradient(xc):
+  """energy.xc_energy (jrx_xc) on a smooth positive density of a non-cubic
+  box: E_xc against oracle.xc.xc_energy (1e-10 relative) and its autograd
+  gradient against oracle.xc.xc_potential (1e-9 relative)."""
+  from jrystal_amd import energy, grid
+  from oracle import xc as OX
+  cell = np.array([[6.0, 0.3, 0.0], [0.0, 5.5, 0.4], [0.2, 0.0, 7.0]])
+  shape = (15, 12, 16)
+  g = grid.g_vectors(cell, shape)
+  vol = abs(np.linalg.det(cell))
+  x = np.stack(np.meshgrid(*[np.arange(n) / n for n in shape], indexing="ij"))
+  rho = (0.05 + 0.03 * np.cos(2 * np.pi * x[0]) * np.sin(2 * np.pi * x[1])
+         + 0.02 * np.cos(2 * np.pi * (x[1] + 2 * x[2])))[None]
+  rt = torch.tensor(rho, device="cuda", requires_grad=True)
+  e = energy.xc_energy(rt, g, vol, xc)
+  e.backward()
+  e_ref = OX.xc_energy(rho, np.asarray(g), vol, xc)
+  v_ref = OX.xc_potential(rho, np.asarray(g), vol, xc)
+  ev = float(e.detach())
+  print(f"{xc}: E_xc {ev:.14f} oracle {e_ref:.14f}", flush=True)
+  assert abs(ev - e_ref) <= 1e-10 * abs(e_ref)
+  assert rel_max(rt.grad.cpu().numpy(), v_ref) <= 1e-9
+
+
+def test_public_xc_energy_rejects_spin_for_gga():
+  from jrystal_amd import energy, grid
+  g = grid.g_vectors(np.eye(3) * 5.0, (8, 8, 8))
+  rho = torch.full((2, 8, 8, 8), 0.01, dtype=torch.float64, device="cuda")
+  with pytest.raises(NotImplementedError):
+    energy.xc_energy(rho, g, 125.0, "gga_x_pbe")
