@@ -206,10 +206,10 @@ __global__ __launch_bounds__(RowCfg<N>::RPB * 8) void row_fft_gather_kernel(
 // first 64 rows are the column's exchange slots); lanes (line y, j) own
 // row y in the x transforms (its 64 slots are the line's exchange slots,
 // used once the line's inputs are in registers).  CP column passes cover
-// nxa <= CP * BLOCK / 8.
-template <int N, int CP>
+// nxa <= CP * BLK / 8.
+template <int N, int CP, int BLK = ZCfg<N>::BLOCK>
 struct ZPlane {
-  static constexpr int BLOCK = ZCfg<N>::BLOCK, V = N / 8, LPB = BLOCK / 8;
+  static constexpr int BLOCK = BLK, V = N / 8, LPB = BLOCK / 8;
   static constexpr int YP = N / LPB;  // line passes
   // Prefetch of the next state's column inputs into registers while the
   // current state is transformed.
@@ -331,14 +331,13 @@ struct ZPlane {
   }
 };
 
-template <int N, int CP>
-__global__ __launch_bounds__(ZCfg<N>::BLOCK,
-                            ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
+template <int N, int CP, int BLK>
+__global__ __launch_bounds__(BLK, ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
 zplane_density_kernel(
     const double2* __restrict__ R, double* __restrict__ rho_part,
     const double* __restrict__ occ, int64_t state0, int nstates,
     double inv_vol, BoxGeom geo, const double2* __restrict__ tw) {
-  using Z = ZPlane<N, CP>;
+  using Z = ZPlane<N, CP, BLK>;
   constexpr int BLOCK = Z::BLOCK, V = Z::V, LPB = Z::LPB, YP = Z::YPD;
   extern __shared__ double2 sm[];  // P [N][kZRow], twiddles [N]
   double2* stw = sm + N * kZRow;
@@ -403,13 +402,12 @@ zplane_density_kernel(
   }
 }
 
-template <int N, int CP>
-__global__ __launch_bounds__(ZCfg<N>::BLOCK,
-                            ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
+template <int N, int CP, int BLK>
+__global__ __launch_bounds__(BLK, ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
 zplane_vpsi_kernel(
     double2* __restrict__ R, const double* __restrict__ vpot, int nstates,
     BoxGeom geo, const double2* __restrict__ tw) {
-  using Z = ZPlane<N, CP>;
+  using Z = ZPlane<N, CP, BLK>;
   constexpr int BLOCK = Z::BLOCK, V = Z::V, LPB = Z::LPB, YP = Z::YP;
   extern __shared__ double2 sm[];
   double2* stw = sm + N * kZRow;
@@ -592,15 +590,15 @@ int launch_row_fft_gather(const double2* R, const double2* q, double2* qbar,
                                  geo, tw, gp, s);
 }
 
-template <int N, int CP>
+template <int N, int CP, int BLK = ZCfg<N>::BLOCK>
 static int zdens_n(const double2* R, double* rho_part, int groups,
                    const double* occ, int64_t state0, int nstates,
                    double inv_vol, const BoxGeom& geo, const double2* tw,
                    hipStream_t s) {
   const size_t shm = (size_t)(N * kZRow + N) * sizeof(double2);
-  hipLaunchKernelGGL((zplane_density_kernel<N, CP>),
-                     dim3(N, groups, ZPlane<N, CP>::YS),
-                     dim3(ZCfg<N>::BLOCK), shm, s, R, rho_part, occ, state0,
+  hipLaunchKernelGGL((zplane_density_kernel<N, CP, BLK>),
+                     dim3(N, groups, ZPlane<N, CP, BLK>::YS),
+                     dim3(BLK), shm, s, R, rho_part, occ, state0,
                      nstates, inv_vol, geo, tw);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
@@ -620,16 +618,19 @@ int launch_zplane_density(const double2* R, double* rho_part, int groups,
   if (geo.nxa <= ZCfg<64>::BLOCK / 8)
     return zdens_n<64, 1>(R, rho_part, groups, occ, state0, nstates, inv_vol,
                           geo, tw, s);
-  return zdens_n<64, 2>(R, rho_part, groups, occ, state0, nstates, inv_vol,
-                        geo, tw, s);
+  // more active columns than a 256-thread block has lane groups: a
+  // 512-thread block (one column pass; a second pass of 256-thread blocks
+  // doubled the registers and halved the occupancy)
+  return zdens_n<64, 1, 512>(R, rho_part, groups, occ, state0, nstates,
+                             inv_vol, geo, tw, s);
 }
 
-template <int N, int CP>
+template <int N, int CP, int BLK = ZCfg<N>::BLOCK>
 static int zvpsi_n(double2* R, const double* v, int groups, int nstates,
                    const BoxGeom& geo, const double2* tw, hipStream_t s) {
   const size_t shm = (size_t)(N * kZRow + N) * sizeof(double2);
-  hipLaunchKernelGGL((zplane_vpsi_kernel<N, CP>), dim3(N, groups),
-                     dim3(ZCfg<N>::BLOCK), shm, s, R, v, nstates, geo, tw);
+  hipLaunchKernelGGL((zplane_vpsi_kernel<N, CP, BLK>), dim3(N, groups),
+                     dim3(BLK), shm, s, R, v, nstates, geo, tw);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -643,7 +644,7 @@ int launch_zplane_vpsi(double2* R, const double* v, int groups, int nstates,
   if (geo.nx == 128) return zvpsi_n<128, 1>(R, v, groups, nstates, geo, tw, s);
   if (geo.nxa <= ZCfg<64>::BLOCK / 8)
     return zvpsi_n<64, 1>(R, v, groups, nstates, geo, tw, s);
-  return zvpsi_n<64, 2>(R, v, groups, nstates, geo, tw, s);
+  return zvpsi_n<64, 1, 512>(R, v, groups, nstates, geo, tw, s);
 }
 
 // out[x][y][z] = in[z][y][x] for one (s) box of n^3 (32 x 32 tiles).

@@ -107,3 +107,18 @@ def test_row_path_128_gamma_si64():
   _check_oracle(out, wl, p)
   with pytest.raises(NotImplementedError, match="row-buffer path"):
     _plan(wl, fft_variant=2)  # a 128 x 128 plane does not fit the LDS
+
+
+def test_row_path_more_than_32_active_planes():
+  """The strained 8-atom cell has 35 active x-planes at 64^3: its z-plane
+  passes run 512-thread blocks (64 column lane groups)."""
+  from jrystal_amd import workloads as W
+  wl = W.si8_strained(4.0, k=2)
+  plan = _plan(wl)
+  assert plan.row_path and plan.active_planes > 32
+  p = seeded_params(wl.shape, 13)
+  out = _run(plan, p, band=True)
+  _check_oracle(out, wl, p)
+  ref = _run(_plan(wl, fft_variant=2), p, band=True)
+  _compare(out, ref)
+  assert rel_max(out["band"], ref["band"]) <= 1e-11
