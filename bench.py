@@ -20,11 +20,15 @@ path runs on one GPU), --steps K, --warmup W, --no-cpu-baseline,
 --no-batch, --batch-steps K (the uneven-structure batch section),
 --traffic-probe (the workload of the rocprofv3 counter runs),
 --no-large-cell, --large-cell-steps K, --large-cell-probe (the 128^3 /
-128-band section and its rocprofv3 workload).
+128-band section and its rocprofv3 workload), --no-small-cell.
 
 A third section, "batch", times Adam steps (energy + gradient + update) of
 16 uneven structures run as one batch: eager, over several streams, and as
 a replayed hipGraph.
+
+A fifth, "small_cell", times the reference's own CPU-sized case (2-atom FCC
+Si, 32^3, 8 k, 8 bands) launched eagerly and replayed as a captured HIP
+graph.
 
 A fourth, "large_cell", times evaluations of the 64-atom Si cell at the
 Gamma point (128^3 box, 128 bands) with the FP64 MFMA rate of its QR
@@ -279,6 +283,54 @@ def qr_flops(cls, plan):
   return 8.0 * P * ng * per[cls]
 
 
+def small_cell_bench(steps, warmup):
+  """Energy + gradient evaluations of the reference's own CPU-sized case
+  (2-atom FCC Si, 32^3, 8 k-points, 8 bands, 24 Ha): launched eagerly from
+  the host, and replayed as one captured HIP graph (the case is bound by
+  launch latency)."""
+  import numpy as np
+  import torch
+  from jrystal_amd import workloads
+  from jrystal_amd.objective import EnergyPlan
+  wl = workloads.si2_fcc(14.0)
+  p = wl.params(0)
+  plan = EnergyPlan.from_workload(wl)
+  wr = torch.tensor(p["w_re"], device=plan.device)
+  wi = torch.tensor(p["w_im"], device=plan.device)
+  out = {}
+  print(f"bench: small cell {wl.name}", file=sys.stderr, flush=True)
+  res = {"workload": wl.name, "grid": list(wl.grid_sizes),
+         "num_kpts": int(plan.nk), "num_bands": int(plan.nb),
+         "num_g": int(plan.num_g), "steps": steps}
+  for _ in range(warmup):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    plan.evaluate(wr, wi, grad=True, out=out)
+  torch.cuda.synchronize()
+  dt = (time.perf_counter() - t0) / steps
+  res["eager"] = {"ms_per_eval": dt * 1e3, "evals_per_s": 1.0 / dt}
+  graph, gout = plan.capture(wr, wi)
+  for _ in range(warmup):
+    graph.replay()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    graph.replay()
+  torch.cuda.synchronize()
+  dt = (time.perf_counter() - t0) / steps
+  res["graph"] = {"ms_per_eval": dt * 1e3, "evals_per_s": 1.0 / dt}
+  e = gout["energies"].tolist()
+  if not np.all(np.isfinite(e)) or e != out["energies"].tolist():
+    raise RuntimeError("small cell: graph replay differs from eager")
+  res["energy_total_Ha"] = e[4]
+  del graph
+  plan.close()
+  torch.cuda.empty_cache()
+  return res
+
+
 def large_cell_bench(steps, warmup):
   """Energy + gradient evaluations of the 64-atom Si cell at the Gamma
   point (128^3 box, 128 bands, 48 Ha, 137059 coefficients per band), with
@@ -480,6 +532,7 @@ def main():
   ap.add_argument("--batch-steps", type=int, default=5)
   ap.add_argument("--traffic-probe", action="store_true",
                   help="run the rocprofv3 counter workload and exit")
+  ap.add_argument("--no-small-cell", action="store_true")
   ap.add_argument("--no-large-cell", action="store_true")
   ap.add_argument("--large-cell-steps", type=int, default=5)
   ap.add_argument("--large-cell-probe", action="store_true",
@@ -586,6 +639,10 @@ def main():
   if not args.no_batch:
     batch = batch_bench(args.batch_steps, 1)
 
+  small = None
+  if not args.no_small_cell:
+    small = small_cell_bench(200, 20)
+
   large = None
   if not args.no_large_cell:
     large = large_cell_bench(args.large_cell_steps, 1)
@@ -631,6 +688,7 @@ def main():
     "cpu_baseline": cpu,
     "convergence": conv,
     "batch": batch,
+    "small_cell": small,
     "large_cell": large,
   }
   print(json.dumps(line))

@@ -193,6 +193,26 @@ class EnergyPlan:
     _lib.check(rc)
     return out
 
+  def capture(self, w_re, w_im, grad: bool = True, out: Optional[dict] = None):
+    """One evaluation recorded as a HIP graph (torch.cuda.CUDAGraph): the
+    evaluation of the tensors w_re, w_im (read again at every replay) into
+    the tensors of ``out``.  Returns (graph, out); graph.replay() re-runs
+    every kernel with one launch from the host.  The evaluation launches no
+    memset and makes no host synchronisation, so it records as is; plan
+    timing must be off."""
+    w_re = self._as_param(w_re, "w_re")
+    w_im = self._as_param(w_im, "w_im")
+    out = {} if out is None else out
+    side = torch.cuda.Stream(device=self.device)
+    side.wait_stream(torch.cuda.current_stream(self.device))
+    with torch.cuda.stream(side):  # allocate out and load every kernel
+      self.evaluate(w_re, w_im, grad=grad, out=out)
+    torch.cuda.current_stream(self.device).wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+      self.evaluate(w_re, w_im, grad=grad, out=out)
+    return graph, out
+
   def precondition(self, g_re, g_im, factor=None, out_re=None, out_im=None):
     """Kinetic (Teter-Payne-Allan) preconditioning of a gradient:
     out = K(|G+k|^2 / tb) * factor * g, with the band norms tb of the last
