@@ -11,8 +11,14 @@ Random cells (numpy.random.default_rng(seed)):
   * cube8 -- 8-atom diamond cubes, a = 10.26 bohr, strain amplitude 0.1,
     displacements N(0, 0.15) bohr, Z = 4; 64^3 grid, Gamma or a 2x2x2
     k-mesh, 32 bands, cutoff 48 Ha.
+  * uneven -- cells drawn like the uneven batch of workloads.batch_cells: a
+    cubic or FCC cell of 2-8 atoms (volume per atom U(60, 140) bohr^3),
+    Z from {3, 6, 11, 13, 14}, uniform random positions, a cubic grid from
+    the 7-smooth sizes in [32, 48], a k-mesh of 1^3 or 2^3, 4-24 bands and
+    the grid's alias-free cutoff.
 Mixes of kinds: "small" cycles fcc2, fcc2, fcc2, cube8 (Gamma); "wide"
-cycles fcc2, fcc2, cube8 (Gamma), cube8 (2x2x2).
+cycles fcc2, fcc2, cube8 (Gamma), cube8 (2x2x2); "uneven" cycles fcc2,
+uneven, cube8 (Gamma), fcc2, uneven, cube8 (2x2x2).
 Cells whose minimum-image atom separation is below 2.5 bohr are redrawn.
 Trajectory seeds start at 1000, away from the strained held-out cell
 (strain seed 4).
@@ -41,7 +47,35 @@ def _min_separation(cell, pos):
   return best
 
 
+def uneven_cell(seed: int):
+  """A cell of the uneven-batch kind, at training sizes."""
+  rng = np.random.default_rng(seed)
+  lattice = ("cubic", "fcc")[int(rng.integers(2))]
+  natom = int(rng.integers(2, 9))
+  z = float(rng.choice([3.0, 6.0, 11.0, 13.0, 14.0]))
+  vol = natom * rng.uniform(60.0, 140.0)
+  if lattice == "cubic":
+    cell = vol**(1 / 3) * np.eye(3)
+  else:
+    a = (4 * vol)**(1 / 3)
+    cell = a / 2 * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0.]])
+  while True:
+    pos = rng.random((natom, 3)) @ cell
+    if _min_separation(cell, pos) >= 2.5:
+      break
+  ng = int(rng.choice([n for n in W.SMOOTH_32_80 if n <= 48]))
+  k = int(rng.integers(1, 3))
+  nb = int(rng.integers(4, 25))
+  gs = (ng, ng, ng)
+  cr = Crystal(charges=np.full(natom, z), positions=pos, cell_vectors=cell,
+               spin=0, symbols=["X"] * natom)
+  return W.build(f"uneven-{seed}", cr, gs, (k, k, k), nb,
+                 W.alias_free_cutoff(cell, gs))
+
+
 def random_cell(seed: int, kind: str, k: int = 1):
+  if kind == "uneven":
+    return uneven_cell(seed)
   rng = np.random.default_rng(seed)
   while True:
     strain = W.strain_matrix(int(rng.integers(1 << 30)), 0.1)
@@ -69,6 +103,8 @@ def random_cell(seed: int, kind: str, k: int = 1):
 MIXES = {
   "small": (("fcc2", 1), ("fcc2", 1), ("fcc2", 1), ("cube8", 1)),
   "wide": (("fcc2", 1), ("fcc2", 1), ("cube8", 1), ("cube8", 2)),
+  "uneven": (("fcc2", 1), ("uneven", 1), ("cube8", 1), ("fcc2", 1),
+             ("uneven", 1), ("cube8", 2)),
 }
 
 
