@@ -183,6 +183,13 @@ def convergence(time_cap, tol=1e-6):
   learned.apply(plan, net, warm.stats,
                 gap=learned.band_gaps(plan) if net.use_gap else None,
                 factor=warm.factor)
+  # and short runs of both minimisers (CG vector kernels, gauge resets,
+  # the factor from its first refresh on): the first timed run would
+  # otherwise pay the loading of everything it shares with the others
+  Minimizer(plan, start, restart=PLAIN_RESTART).run(max_iter=40)
+  Minimizer(plan, start, factor_fn=learned.LearnedFactor(plan, net),
+            restart=learned.LEARNED_RESTART,
+            first_restart=learned.LEARNED_FIRST_RESTART).run(max_iter=40)
   torch.cuda.synchronize()
   for mode in ("learned", "learned_warm", "plain"):
     print(f"bench: convergence run {mode}", file=sys.stderr, flush=True)
