@@ -54,14 +54,20 @@ __device__ __forceinline__ void load_small(double2* dst, const double2* src,
 }
 
 // ------------------------------------------------- wide kernels (nb > 64) --
+// Threads of the wide Gram kernel: 512 at 8 tiles (128 bands: 36 or 64
+// pairs over 8 waves, 5 or 8 per wave); below, 256 (the compiler keeps the
+// accumulators in AGPRs there and spills at 512).
+constexpr int gram_wide_block(int T) { return T == 8 ? 512 : 256; }
+
 // Tile pairs (I, J) of the wide Gram kernels: HERM takes I <= J (row-major
 // over the upper triangle; the result is Hermitian), otherwise all T x T.
-// NZ blocks share one chunk's pairs, PW pairs to a wave.
+// NZ blocks share one chunk's pairs, PW pairs to a wave (at most 9).
 template <int T, bool HERM>
 struct WidePairs {
+  static constexpr int NW = gram_wide_block(T) / 64;
   static constexpr int NPAIR = HERM ? T * (T + 1) / 2 : T * T;
-  static constexpr int NZ = (NPAIR + 35) / 36;
-  static constexpr int PW = (NPAIR + 4 * NZ - 1) / (4 * NZ);
+  static constexpr int NZ = (NPAIR + 9 * NW - 1) / (9 * NW);
+  static constexpr int PW = (NPAIR + NW * NZ - 1) / (NW * NZ);
 };
 
 __device__ inline void decode_pair(int q, int T, bool herm, int* I, int* J) {
@@ -79,25 +85,36 @@ __device__ inline void decode_pair(int q, int T, bool herm, int* I, int* J) {
   *J = i + q;
 }
 
-constexpr int kWideSlab = 16;  // rows per LDS slab of the wide Gram kernel
+// Rows per LDS slab of the wide Gram kernel: 16 for a Hermitian result
+// (one staged matrix), 8 for Q^H Qbar (two), so that the two slab buffers
+// fit twice per CU.
+template <bool HERM>
+constexpr int wide_slab() { return HERM ? 16 : 8; }
 
 // Partial Gram sums over a chunk's rows, one partial per chunk:
 //   GMAJOR: S = W^H W (W planar g-major, the Hermitian pairs only);
 //   else:   G = A^H B (A, B band-major complex; HERM when B is A).
-// Row slabs are staged in LDS ([row][NBP + 1]); each wave accumulates its
-// own tile pairs and writes them (and, for HERM, their mirror images).
+// Row slabs are staged in LDS ([row][NBP + 1]), double-buffered: the next
+// slab's global loads are in flight in registers while the MFMAs run on
+// the current one, one barrier per slab.  Each wave accumulates its own
+// tile pairs and writes them (and, for HERM, their mirror images).
 template <int T, bool GMAJOR, bool HERM>
-__global__ __launch_bounds__(kQrBlock) void gram_wide_kernel(
+__global__ __launch_bounds__(gram_wide_block(T)) void gram_wide_kernel(
     const double* __restrict__ wr, const double* __restrict__ wi,
     const double2* __restrict__ qa, const double2* __restrict__ qb,
     double2* __restrict__ partial, int ng, int nb, int rows_per_chunk,
     int nchunk, const int* __restrict__ skip) {
   using PL = WidePairs<T, HERM>;
-  constexpr int NBP = 16 * T, LD = NBP + 1, PW = PL::PW, SR = kWideSlab;
+  constexpr int NBP = 16 * T, LD = NBP + 1, PW = PL::PW;
+  constexpr int SR = wide_slab<HERM>();
+  constexpr int NS = SR * NBP;                        // elements per slab
+  constexpr int BLK = gram_wide_block(T);
+  constexpr int NE = (NS + BLK - 1) / BLK;  // elements per thread
+  constexpr int BUF = (HERM ? 1 : 2) * SR * LD;       // one slab buffer
   extern __shared__ double2 sm[];
-  double2* sa = sm;
-  double2* sb = HERM ? sm : sm + SR * LD;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // (the wave index through readfirstlane: the pair tables are scalars)
+  const int lane = threadIdx.x & 63,
+            wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
   const int64_t p = blockIdx.y;
   const int chunk = blockIdx.x;
@@ -106,7 +123,7 @@ __global__ __launch_bounds__(kQrBlock) void gram_wide_kernel(
   bool live[PW];
 #pragma unroll
   for (int t = 0; t < PW; ++t) {
-    const int q = ((int)blockIdx.z * 4 + wave) * PW + t;
+    const int q = ((int)blockIdx.z * PL::NW + wave) * PW + t;
     live[t] = q < PL::NPAIR;  // uniform over the wave
     decode_pair(live[t] ? q : 0, T, HERM, &pI[t], &pJ[t]);
   }
@@ -119,31 +136,59 @@ __global__ __launch_bounds__(kQrBlock) void gram_wide_kernel(
   const double* Wi = wi + p * ng * nb;
   const double2* A = qa + p * nb * ng;
   const double2* B = qb + p * nb * ng;
-  for (int g0 = gb; g0 < ge; g0 += SR) {
-    __syncthreads();  // the previous slab is consumed
-    for (int e = threadIdx.x; e < SR * NBP; e += kQrBlock) {
-      if (GMAJOR) {  // consecutive threads: consecutive bands of a row
-        const int r = e / NBP, c = e - r * NBP, g = g0 + r;
-        const bool ok = g < ge && c < nb;
-        sa[r * LD + c] = ok ? make_double2(Wr[(int64_t)g * nb + c],
-                                           Wi[(int64_t)g * nb + c])
-                            : czero();
-      } else {  // consecutive threads: consecutive rows of a band
-        const int c = e / SR, r = e - c * SR, g = g0 + r;
-        const bool ok = g < ge && c < nb;
-        sa[r * LD + c] = ok ? A[(int64_t)c * ng + g] : czero();
-        if (!HERM) sb[r * LD + c] = ok ? B[(int64_t)c * ng + g] : czero();
+  double2 ra[NE], rb[HERM ? 1 : NE];
+  // slab element e: GMAJOR row e / NBP, band e % NBP (consecutive threads
+  // read consecutive bands of a row); band-major band e / SR, row e % SR
+  // (consecutive threads read consecutive rows of a band)
+  auto fetch = [&](int g0) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = threadIdx.x + i * BLK;
+      const int r = GMAJOR ? e / NBP : e % SR;
+      const int c = GMAJOR ? e % NBP : e / SR;
+      const int g = g0 + r;
+      const bool ok = e < NS && g < ge && c < nb;
+      if (GMAJOR) {
+        ra[i] = ok ? make_double2(Wr[(int64_t)g * nb + c], Wi[(int64_t)g * nb + c])
+                   : czero();
+      } else {
+        ra[i] = ok ? A[(int64_t)c * ng + g] : czero();
+        if (!HERM) rb[i] = ok ? B[(int64_t)c * ng + g] : czero();
       }
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {
+    double2* sa = sm + buf * BUF;
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = threadIdx.x + i * BLK;
+      if (e < NS) {
+        const int r = GMAJOR ? e / NBP : e % SR;
+        const int c = GMAJOR ? e % NBP : e / SR;
+        sa[r * LD + c] = ra[i];
+        if (!HERM) sa[SR * LD + r * LD + c] = rb[i];
+      }
+    }
+  };
+  if (gb < ge) {
+    fetch(gb);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int g0 = gb; g0 < ge; g0 += SR) {
+    const bool more = g0 + SR < ge;  // uniform over the block
+    if (more) fetch(g0 + SR);
+    const double2* sa = sm + buf * BUF;
+    const double2* sb = HERM ? sa : sa + SR * LD;
 #pragma unroll
     for (int r = 0; r < SR / 4; ++r) {
-      const double2* ra = sa + (4 * r + lk) * LD + li;
-      const double2* rb = sb + (4 * r + lk) * LD + li;
+      const double2* ra4 = sa + (4 * r + lk) * LD + li;
+      const double2* rb4 = sb + (4 * r + lk) * LD + li;
 #pragma unroll
       for (int t = 0; t < PW; ++t) {
         if (!live[t]) continue;
-        const double2 a = ra[16 * pI[t]], b = rb[16 * pJ[t]];
+        const double2 a = ra4[16 * pI[t]], b = rb4[16 * pJ[t]];
         // conj(a) b
         re[t] = mfma4(a.x, b.x, re[t]);
         re[t] = mfma4(a.y, b.y, re[t]);
@@ -151,6 +196,10 @@ __global__ __launch_bounds__(kQrBlock) void gram_wide_kernel(
         im[t] = mfma4(-a.y, b.x, im[t]);
       }
     }
+    // the other buffer was last read before the previous barrier
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
   double2* out = partial + (p * nchunk + chunk) * nb * nb;
 #pragma unroll
@@ -176,13 +225,15 @@ int launch_gram_wide(const double* wr, const double* wi, const double2* qa,
                      const int* skip) {
   const int T = (nb + 15) / 16;
   const bool gmajor = wr != nullptr, herm = gmajor || qa == qb;
-  const size_t shm =
-      (herm ? 1 : 2) * (size_t)kWideSlab * (16 * T + 1) * sizeof(double2);
-  dim3 block(kQrBlock);
+  // two slab buffers of one (HERM) or two staged matrices
+  const size_t shm = 2 * (herm ? 1 : 2) *
+                     (size_t)(herm ? wide_slab<true>() : wide_slab<false>()) *
+                     (16 * T + 1) * sizeof(double2);
 #define JRX_GRAM_WIDE(TT, GM, HM)                                            \
   hipLaunchKernelGGL((gram_wide_kernel<TT, GM, HM>),                         \
                      dim3(sp.nchunk, (unsigned)P, WidePairs<TT, HM>::NZ),    \
-                     block, shm, s, wr, wi, qa, qb, partial, ng, nb,         \
+                     dim3(gram_wide_block(TT)), shm, s, wr, wi, qa, qb,     \
+                     partial, ng, nb,                                        \
                      sp.rows_per_chunk, sp.nchunk, skip)
 #define JRX_GRAM_WIDE_T(TT)                              \
   if (gmajor) JRX_GRAM_WIDE(TT, true, true);             \
@@ -1095,7 +1146,7 @@ size_t qr_lds_bytes(int nb) {
   const size_t nbp = 16 * (size_t)((nb + 15) / 16), z = sizeof(double2);
   if (nb <= kNarrowBands)  // Gram (pre + sums), apply_bwd, chol_inv
     return std::max(2 * nbp * nbp * z, 2 * (size_t)nb * (nb + 1) * z);
-  const size_t gram = 2 * (size_t)kWideSlab * (nbp + 1) * z;
+  const size_t gram = 2 * 2 * (size_t)wide_slab<false>() * (nbp + 1) * z;
   const size_t fwd = (size_t)kWideK * nbp * z;
   const size_t bwd = 2 * (size_t)kWideKB * nbp * z;
   const size_t chol = ((size_t)nb * (nb + 1) / 2 + 2 * nb) * z +
