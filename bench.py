@@ -421,7 +421,7 @@ def measured_mfma():
 
 
 def large_cell_probe():
-  """Workload of the large-cell rocprofv3 runs: two energy + gradient
+  """Workload of the large-cell rocprofv3 runs: eight energy + gradient
   evaluations of the 64-atom Gamma cell."""
   import torch
   from jrystal_amd import workloads
@@ -432,7 +432,7 @@ def large_cell_probe():
   wr = torch.tensor(p["w_re"], device="cuda")
   wi = torch.tensor(p["w_im"], device="cuda")
   out = {}
-  for _ in range(2):
+  for _ in range(8):
     plan.evaluate(wr, wi, grad=True, out=out)
   torch.cuda.synchronize()
   print(f"large cell probe done: E = {float(out['energies'][4]):.10f}",
@@ -446,9 +446,9 @@ KERNEL_NAMES = {"xline_vpsi": "jrx::xline_vpsi64_kernel",
                 "plane_ifft": "jrx::plane_ifft64_kernel",
                 "xline_density": "jrx::xline_density64_kernel",
                 "plane_fft_gather": "jrx::plane_fft_gather64_kernel",
-                "zplane_vpsi": "jrx::zplane_vpsi_kernel<64, 1>",
+                "zplane_vpsi": "jrx::zplane_vpsi_kernel<64, 1, 256>",
                 "row_ifft": "jrx::row_ifft_kernel<64>",
-                "zplane_density": "jrx::zplane_density_kernel<64, 1>",
+                "zplane_density": "jrx::zplane_density_kernel<64, 1, 256>",
                 "row_fft_gather": "jrx::row_fft_gather_kernel<64, false>"}
 # On the row-buffer path the four FFT timing classes hold these kernels.
 ROW_PATH_CLASSES = {"plane_ifft": "row_ifft",
@@ -497,7 +497,8 @@ def measured_traffic(cls):
 def traffic_probe():
   """Workload of the rocprofv3 counter runs: a calibration kernel of known
   traffic (jrx_vec_axpby on 2^27 float64: reads 2 GiB, writes 1 GiB), then
-  two energy+gradient evaluations of the benchmark workload."""
+  twelve energy+gradient evaluations of the benchmark workload (enough for
+  the clocks to settle, so the traced durations match the live ones)."""
   import ctypes
   import torch
   from jrystal_amd import _lib, workloads
@@ -518,7 +519,7 @@ def traffic_probe():
   wr = torch.tensor(p["w_re"], device="cuda")
   wi = torch.tensor(p["w_im"], device="cuda")
   out = {}
-  for _ in range(2):
+  for _ in range(12):
     plan.evaluate(wr, wi, grad=True, out=out)
   torch.cuda.synchronize()
   print(f"traffic probe done: E = {float(out['energies'][4]):.10f}",
