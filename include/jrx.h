@@ -191,6 +191,15 @@ int jrx_plan_subspace_matrix(const jrx_plan* plan, double* out,
 int jrx_plan_rotate(const jrx_plan* plan, const double* v, double* out_re,
                     double* out_im, const void* workspace,
                     size_t workspace_bytes, void* stream);
+/* jrx_plan_ritz_rotation: after jrx_energy_grad with a gradient, v [s][k]
+ *   (complex nb x nb) = the Rayleigh-Ritz rotation of a gauge reset: on each
+ *   run of >= 2 consecutive bands with equal occupation f > 0, the
+ *   eigenvectors (ascending eigenvalues) of the Hermitised subspace
+ *   Hamiltonian (Q^H Qbar) / (2 f) over the run, by cyclic Jacobi on the
+ *   device; the identity elsewhere.  No counterpart in the reference (its
+ *   minimiser is Adam); replaces a host eigendecomposition. */
+int jrx_plan_ritz_rotation(const jrx_plan* plan, double* v, void* workspace,
+                           size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner and warm start (no counterpart in the reference).

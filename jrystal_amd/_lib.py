@@ -116,6 +116,7 @@ SIGNATURES = {
   "jrx_plan_orthonormal_params": [_P, _P, _P, _P, _SZ, _P],
   "jrx_plan_subspace_matrix": [_P, _P, _P, _SZ, _P],
   "jrx_plan_rotate": [_P, _P, _P, _P, _P, _SZ, _P],
+  "jrx_plan_ritz_rotation": [_P, _P, _P, _SZ, _P],
   "jrx_plan_density_stats": [_P, _P, _P, _SZ, _P],
   "jrx_plan_features": [_P, _P, _I64, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_mlp_apply": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],

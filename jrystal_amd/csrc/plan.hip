@@ -130,6 +130,7 @@ struct jrx_plan {
   std::vector<std::pair<int, int>> records;  // (class, event pair index)
   // workspace offsets (bytes)
   size_t o_gb = 0;  // density-gradient boxes of GGA terms [3][N] complex
+  size_t o_ritz = 0;  // scratch of the Rayleigh-Ritz rotation
   size_t o_stats;
   size_t o_q, o_qbar, o_T, o_rho, o_rpart, o_v, o_box, o_tb, o_S, o_R1, o_R2, o_Ri, o_A,
       o_Bm, o_part, o_kin, o_eh, o_ee, o_exc, o_qrflag, o_bepart, o_pot,
@@ -435,6 +436,7 @@ int plan_build(jrx_plan* p, const jrx_problem* d) {
   p->o_qrflag = take((size_t)p->P * sizeof(int));
   p->o_bepart = take((size_t)kin_gather * sizeof(double));
   p->o_pot = take((size_t)p->S * sizeof(double));
+  p->o_ritz = take(ritz_scratch_bytes(p->P, nb));
   p->total = off;
   return JRX_OK;
 }
@@ -824,6 +826,17 @@ extern "C" int jrx_plan_subspace_matrix(const jrx_plan* p, double* out,
       (size_t)p->P * p->nb * p->nb * sizeof(double2),
       hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return JRX_OK;
+}
+
+extern "C" int jrx_plan_ritz_rotation(const jrx_plan* p, double* v,
+                                      void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  JRX_REQUIRE(p && v && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  char* ws = static_cast<char*>(workspace);
+  return launch_ritz(reinterpret_cast<const double2*>(ws + p->o_S), p->d_occ,
+                     p->P, p->nb, reinterpret_cast<double2*>(v),
+                     ws + p->o_ritz, (hipStream_t)stream);
 }
 
 extern "C" int jrx_plan_rotate(const jrx_plan* p, const double* v,

@@ -81,4 +81,12 @@ int launch_apply_bwd(const double2* qbar, const double2* q, const double2* A,
                      const double2* Bm, double* dwr, double* dwi, int64_t P,
                      int ng, int nb, hipStream_t s);
 
+// Rayleigh-Ritz rotation (ritz.hip): V[p] = eigenvectors, ascending, of
+// the Hermitised (S[p] / (2 f)) on each run of >= 2 consecutive bands with
+// equal occupation f > 0, identity elsewhere.  scratch:
+// ritz_scratch_bytes(P, nb), 256-B aligned.
+size_t ritz_scratch_bytes(int64_t P, int nb);
+int launch_ritz(const double2* S, const double* occ, int64_t P, int nb,
+                double2* V, void* scratch, hipStream_t s);
+
 }  // namespace jrx

@@ -114,6 +114,7 @@ class Minimizer:
     self.flat_window = 12
     self.flat_tol = 1e-14
     self._s0_prev = None
+    self._ritz_v = None
     self.evals = 0
 
   # -- kernels -------------------------------------------------------------
@@ -155,30 +156,16 @@ class Minimizer:
       y.data_ptr(), out.data_ptr(), self.n, _lib.stream_ptr()))
 
   def _ritz_rotation(self):
-    """Unitary V (s,k,nb,nb): on each run of consecutive bands with equal
-    occupation f > 0, the eigenvectors of the subspace Hamiltonian
-    (Q^H Qbar) / 2f in ascending order; identity elsewhere.  Needs the
-    last evaluation to carry a gradient."""
-    S = self.plan.subspace_matrix().cpu().numpy()
-    occ = self.plan.occupation
-    ns, nk, nb = occ.shape
-    V = np.broadcast_to(np.eye(nb, dtype=complex), S.shape).copy()
-    rows = occ.reshape(-1, nb)
-    Sf = S.reshape(-1, nb, nb)
-    Vf = V.reshape(-1, nb, nb)
-    for pattern in np.unique(rows, axis=0):
-      sel = np.nonzero(np.all(rows == pattern, axis=1))[0]
-      b = 0
-      while b < nb:
-        e = b
-        while e + 1 < nb and pattern[e + 1] == pattern[b]:
-          e += 1
-        if pattern[b] > 0 and e > b:
-          h = Sf[sel, b:e + 1, b:e + 1] / (2.0 * pattern[b])
-          h = 0.5 * (h + np.conj(np.swapaxes(h, -1, -2)))
-          Vf[sel, b:e + 1, b:e + 1] = np.linalg.eigh(h)[1]
-        b = e + 1
-    return torch.from_numpy(V)
+    """Unitary V (s,k,nb,nb) on the device: on each run of consecutive
+    bands with equal occupation f > 0, the eigenvectors of the subspace
+    Hamiltonian (Q^H Qbar) / 2f in ascending order; identity elsewhere
+    (a GPU Jacobi eigensolver, no host round trip).  Needs the last
+    evaluation to carry a gradient."""
+    if self._ritz_v is None:
+      self._ritz_v = torch.empty(
+        (self.plan.ns, self.plan.nk, self.plan.nb, self.plan.nb),
+        dtype=torch.complex128, device=self.plan.device)
+    return self.plan.ritz_rotation(self._ritz_v)
 
   def _restart(self, gh_before=None):
     """Gauge reset W <- Q V of the last evaluation (same energy; R = I and,

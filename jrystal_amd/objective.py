@@ -289,6 +289,21 @@ class EnergyPlan:
         self.workspace_bytes, _lib.stream_ptr(self.device)))
     return out_re, out_im
 
+  def ritz_rotation(self, out=None):
+    """Rayleigh-Ritz rotation of the last evaluation with a gradient,
+    (ns,nk,nb,nb) complex128 on the device: on each run of consecutive
+    bands with equal occupation f > 0 the eigenvectors (ascending) of the
+    subspace Hamiltonian (Q^H Qbar) / (2 f), the identity elsewhere
+    (jrx_plan_ritz_rotation, cyclic Jacobi on the GPU)."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb, self.nb),
+                        dtype=torch.complex128, device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_ritz_rotation(
+        self._plan, _lib.ptr(out), self._ws_ptr, self.workspace_bytes,
+        _lib.stream_ptr(self.device)))
+    return out
+
   def subspace_matrix(self, out=None):
     """Q^H Qbar of the last evaluation with a gradient, (ns,nk,nb,nb)
     complex128 on the device; for bands of equal occupation f > 0 it is

@@ -168,7 +168,7 @@ def test_rayleigh_ritz_rotation_keeps_energy_and_diagonalises():
   r = plan.evaluate(p["w_re"], p["w_im"], grad=True)
   e0 = float(r["energies"][4])
   m = Minimizer(plan, p)
-  V = m._ritz_rotation().numpy()
+  V = m._ritz_rotation().cpu().numpy()
   nocc = int((plan.occupation[0, 0] > 0).sum())
   eye = np.eye(plan.nb)
   assert np.max(np.abs(np.conj(np.swapaxes(V, -1, -2)) @ V - eye)) < 1e-12
