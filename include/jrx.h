@@ -358,6 +358,16 @@ int jrx_real_to_complex(const double* x, double* y, int64_t n, void* stream);
 int jrx_complex_real(const double* x, double* y, int64_t n, void* stream);
 int jrx_scale(const double* x, const double* alpha, double* y, int64_t n,
               void* stream);
+/* occupation.simplex_projector's projection (occupation.py:295-366) of
+ * `channels` spin channels of n entries each (in [c][n]): x = sigmoid(in)
+ * when sigmoid != 0, else in; y = the point of {0 <= y <= 1, sum y =
+ * totals[c]} reached by one uniform shift and clipping, with the
+ * reference's sorted-cumulative-sum rule for the shift.  With ybar != NULL
+ * also in_bar = d(sum ybar * y) / d in (active set fixed).  n <= 8192;
+ * totals is a device array [channels]. */
+int jrx_simplex_projection(const double* in, int64_t n, int channels,
+                           const double* totals, int sigmoid, double* y,
+                           const double* ybar, double* in_bar, void* stream);
 /* One optax.adam step (opt_utils.py:153-168 -> optax.adam) in place on a
  * flat float64 parameter; m, v are the moment buffers, step is 1-based. */
 int jrx_adam_update(double* param, const double* grad, double* m, double* v,

@@ -103,6 +103,7 @@ SIGNATURES = {
   "jrx_xc_workspace_bytes": [_I, _I, _I, _I],
   "jrx_xc": [_P, _P, _I, _I, _I, _D, _I, _P, _P, _P, _P, _SZ, _P],
   "jrx_adam_update": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _D, _I64, _P],
+  "jrx_simplex_projection": [_P, _I64, _I, _P, _I, _P, _P, _P, _P],
   "jrx_real_to_complex": [_P, _P, _I64, _P],
   "jrx_complex_real": [_P, _P, _I64, _P],
   "jrx_scale": [_P, _P, _P, _I64, _P],

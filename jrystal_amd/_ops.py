@@ -309,6 +309,36 @@ class ExchangeLDA(torch.autograd.Function):
     return grad, None
 
 
+class SimplexProjection(torch.autograd.Function):
+  """Rows of x (channels, n) projected onto {0 <= y <= 1, sum y = total_c}
+  by the reference's shift-and-clip rule (occupation.py:295-337), after a
+  sigmoid when ``sigmoid``; jrx_simplex_projection forward and VJP."""
+
+  @staticmethod
+  def forward(ctx, x, totals, sigmoid):
+    x = x.contiguous()
+    c, n = x.shape
+    tot = torch.as_tensor(totals, dtype=torch.float64).to(x.device)
+    y = torch.empty_like(x)
+    _lib.check(_L().jrx_simplex_projection(_p(x), n, c, _p(tot),
+                                           int(bool(sigmoid)), _p(y), None,
+                                           None, _s()))
+    ctx.save_for_backward(x, tot)
+    ctx.sigmoid = bool(sigmoid)
+    return y
+
+  @staticmethod
+  def backward(ctx, g):
+    x, tot = ctx.saved_tensors
+    c, n = x.shape
+    y = torch.empty_like(x)
+    xb = torch.empty_like(x)
+    _lib.check(_L().jrx_simplex_projection(_p(x), n, c, _p(tot),
+                                           int(ctx.sigmoid), _p(y),
+                                           _p(g.contiguous()), _p(xb), _s()))
+    return xb, None, None
+
+
 class ExchangeCorrelation(torch.autograd.Function):
   """E_xc of the JRX_XC_* terms in `flags` on an unpolarised density
   (1, nx, ny, nz) with the G grid (nx, ny, nz, 3): jrx_xc forward and

@@ -11,6 +11,8 @@ map through a real QR.
 import numpy as np
 import torch
 
+from . import _ops
+
 
 def uniform(num_k, num_electrons, spin=0, num_bands=None, spin_restricted=True):
   """occupation.py:147-190 (silently clips when Ne/2 > num_bands)."""
@@ -53,47 +55,29 @@ def simplex_projector_init(num_bands, num_kpts, device=None):
 
 def proj(x, total):
   """occupation.py:295-337: the point of {0 <= y <= 1, sum y = total}
-  reached from x (1-D, entries in [0, 1]) by a uniform shift and clipping:
-  shift down and clip at 0 when sum x > total, else shift up and clip at 1.
-  The shift follows from the sorted entries and their cumulative sums with
-  the same active-set rule as the reference; only the branch taken is
-  evaluated (the reference evaluates both and selects one, which gives the
-  same values and gradients)."""
-  n = x.shape[0]
-  ar = torch.arange(n, dtype=x.dtype, device=x.device)
-  x_sum = x.sum()
-  with torch.no_grad():
-    up = bool(x_sum <= total)
-  if not up:
-    xs, _ = torch.sort(x)
-    cs = torch.cumsum(xs, 0)
-    rho = (x_sum - total - cs) / (n - ar - 1)
-    with torch.no_grad():
-      k = int(torch.argmax((xs > rho).to(torch.int8)))
-    lam = (x_sum - total) / n if k == 0 else \
-        (x_sum - total - cs[k - 1]) / (n - k)
-    return torch.clamp(x - lam, min=0.0)
-  xs, _ = torch.sort(x, descending=True)
-  cs = torch.cumsum(xs, 0)
-  rho = (x_sum - total + (ar + 1 - cs)) / (n - ar - 1)
-  with torch.no_grad():
-    k = int(torch.argmax((xs - rho < 1.0).to(torch.int8)))
-  lam = (total - x_sum) / n if k == 0 else \
-      (total - x_sum - (k - cs[k - 1])) / (n - k)
-  return torch.clamp(x + lam, max=1.0)
+  reached from x (1-D device tensor, entries in [0, 1]) by a uniform shift
+  and clipping: shift down and clip at 0 when sum x > total, else shift up
+  and clip at 1.  The shift follows from the sorted entries and their
+  cumulative sums with the same active-set rule as the reference (HIP
+  kernel jrx_simplex_projection; differentiable, with the active set held
+  fixed as JAX does for the reference)."""
+  return _ops.SimplexProjection.apply(x.reshape(1, -1), [float(total)],
+                                      False).reshape(x.shape)
 
 
 def simplex_projector(params, num_electrons, spin=0, spin_restricted=True):
   """occupation.py:340-366: occupation (spin, kpt, band) from sigmoid of
   the parameters projected per spin channel onto sum = electrons * kpts,
-  0 <= f <= 1, divided by the number of k-points."""
+  0 <= f <= 1, divided by the number of k-points (both channels in one
+  launch of the projection kernel)."""
   num_kpts, num_bands = params["param_up"].shape
   m_up = (num_electrons + spin) // 2 * num_kpts
   m_down = (num_electrons - spin) // 2 * num_kpts
-  occ_up = proj(torch.sigmoid(params["param_up"]).reshape(-1), m_up)
-  occ_down = proj(torch.sigmoid(params["param_down"]).reshape(-1), m_down)
-  occ = torch.stack([occ_up.reshape(num_kpts, num_bands),
-                     occ_down.reshape(num_kpts, num_bands)]) / num_kpts
+  theta = torch.stack([params["param_up"].reshape(-1),
+                       params["param_down"].reshape(-1)])
+  occ = _ops.SimplexProjection.apply(theta, [float(m_up), float(m_down)],
+                                     True)
+  occ = occ.reshape(2, num_kpts, num_bands) / num_kpts
   if spin_restricted:
     return occ.sum(0, keepdim=True)
   return occ
