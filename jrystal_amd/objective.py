@@ -273,7 +273,19 @@ class EnergyPlan:
       _lib.check(self._lib.jrx_plan_set_occupation(
         self._plan, _lib.ptr(occ), _lib.stream_ptr(self.device)))
     self._occ_dev = occ  # keep the source alive until the copy has run
-    self.occupation = occ.detach().cpu().numpy()
+    self._occ_host = None  # host copy made on first access (no sync here)
+
+  @property
+  def occupation(self):
+    """The occupation in force (ns,nk,nb), a host numpy array; after
+    set_occupation it is copied from the device on first access."""
+    if self._occ_host is None:
+      self._occ_host = self._occ_dev.detach().cpu().numpy()
+    return self._occ_host
+
+  @occupation.setter
+  def occupation(self, value):
+    self._occ_host = np.asarray(value, np.float64)
 
   def orthonormal_params(self, out_re=None, out_im=None):
     """Q of the last evaluation in the (s,k,g,b) parameter layout (real and
