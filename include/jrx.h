@@ -202,6 +202,42 @@ int jrx_plan_ritz_rotation(const jrx_plan* plan, double* v, void* workspace,
                            size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Kohn-Sham band mode (the band-structure driver's objective).
+ *
+ * jrx_plan_set_fixed_density: rho [s][x][y][z] (device, the plan's ns
+ *   channels) -> the effective potential v_eff = V_H + V_ext + v_xc of that
+ *   density (Kohn-Sham form: full Hartree potential, v_xc = d(rho e_xc) /
+ *   d rho), kept by the plan; from then on jrx_energy_grad evaluates
+ *   E = sum_{s,k,b} f <psi|T + v_eff|psi> (energies = {kinetic part,
+ *   potential part, 0, 0, E}) and its gradient, with no density pass.
+ *   rho = NULL returns the plan to the ground-state objective.  The first
+ *   call allocates the potential (ns * nx * ny * nz doubles).  Replaces
+ *   hamiltonian_matrix_trace (jrystal/_src/hamiltonian.py:105-168) with the
+ *   potential of potential.effective (potential.py:203-279, kohn_sham=True).
+ * jrx_plan_set_kpoints: kpts [nk][3] (host or device, 1/bohr) replace the
+ *   plan's k-points for later evaluations (the k-path loop of
+ *   jrystal/calc/calc_band_structure_all_electrons.py:120-152).
+ * jrx_plan_subspace_hamiltonian: after an evaluation (with or without a
+ *   gradient in band mode; with a gradient otherwise), h [s][k] (complex
+ *   nb x nb) = q_i^H H q_j, H = T + v of the evaluation; NaN where an
+ *   occupation is <= 0.  Replaces hamiltonian_matrix (hamiltonian.py:171-240).
+ * jrx_plan_subspace_eigenvalues: the same matrix's eigenvalues, ascending,
+ *   evals [s][k][nb], and optionally its eigenvectors vecs [s][k] (complex
+ *   nb x nb, columns), by cyclic Jacobi on the device; NaN eigenvalues for
+ *   a (s,k) with an occupation <= 0.  Replaces jnp.linalg.eigvalsh of
+ *   calc_band_structure_all_electrons.py:156-169. */
+int jrx_plan_set_fixed_density(jrx_plan* plan, const double* rho,
+                               void* workspace, size_t workspace_bytes,
+                               void* stream);
+int jrx_plan_set_kpoints(jrx_plan* plan, const double* kpts, void* stream);
+int jrx_plan_subspace_hamiltonian(const jrx_plan* plan, double* h,
+                                  const void* workspace,
+                                  size_t workspace_bytes, void* stream);
+int jrx_plan_subspace_eigenvalues(const jrx_plan* plan, double* evals,
+                                  double* vecs, void* workspace,
+                                  size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Learned preconditioner and warm start (no counterpart in the reference).
  * A FP32 MLP, 8 feature slots -> tanh 64 -> tanh 64 -> 2 outputs, with
  * flat parameters (jrx_mlp_num_params() floats: W1[64][8], b1[64],

@@ -118,6 +118,10 @@ SIGNATURES = {
   "jrx_plan_subspace_matrix": [_P, _P, _P, _SZ, _P],
   "jrx_plan_rotate": [_P, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_ritz_rotation": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_set_fixed_density": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_set_kpoints": [_P, _P, _P],
+  "jrx_plan_subspace_hamiltonian": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_subspace_eigenvalues": [_P, _P, _P, _P, _SZ, _P],
   "jrx_plan_density_stats": [_P, _P, _P, _SZ, _P],
   "jrx_plan_features": [_P, _P, _I64, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_mlp_apply": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],

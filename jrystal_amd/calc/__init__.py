@@ -1,6 +1,8 @@
-"""Drivers (jrystal/calc/__init__.py:17-20).  Only the all-electron total
-energy driver is on this path; band-structure and norm-conserving drivers
-are outside it."""
+"""Drivers (jrystal/calc/__init__.py:17-20): the all-electron total energy
+and band-structure drivers; the norm-conserving drivers are outside this
+path."""
+from .band_all_electrons import BandStructureOutput  # noqa: F401
+from .band_all_electrons import calc as band_all_electrons  # noqa: F401
 from .energy_all_electrons import GroundStateEnergyOutput  # noqa: F401
 from .energy_all_electrons import calc as energy_all_electrons  # noqa: F401
 
@@ -13,5 +15,4 @@ def _not_on_path(name):
 
 
 energy_normcons = _not_on_path("energy_normcons")
-band_all_electrons = _not_on_path("band_all_electrons")
 band_normcons = _not_on_path("band_normcons")

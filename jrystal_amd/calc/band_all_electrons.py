@@ -1,0 +1,163 @@
+"""All-electron band-structure driver on the GPU.
+
+Mirrors jrystal/calc/calc_band_structure_all_electrons.py:37-209:
+
+1. the k-path of the cell (band.get_k_path with k_path_special_points and
+   num_kpoints; or the fractional rows of k_path_file, a .npy array);
+2. the ground-state density from the energy driver (same config);
+3. ceil(num_electron / 2) + band_structure_empty_bands bands per spin,
+   initialised by pw.param_init with the config's seed and one k-point;
+4. for the first k-point band_structure_epoch Adam steps on
+   hamiltonian_matrix_trace (all band weights one), then for every
+   following k-point k_path_fine_tuning_epoch steps warm-started from the
+   previous k-point's parameters and Adam state (the reference's scan
+   carries both, :120-152);
+5. after each k-point the ascending eigenvalues of q_i^H H q_j (:154-182);
+6. eigenvalues (spin, k, band) saved as <symbols>_band_structure.npy
+   (:199-209), in save_dir when it is set.
+
+One plan in band mode serves the whole path (set_kpoints moves it along);
+an Adam step is one fused evaluation plus the fused Adam kernel, with no
+host synchronisation.  optimizer: cg replaces the Adam steps at each
+k-point by preconditioned conjugate gradients on the same objective, at
+most as many iterations as the Adam schedule has steps.
+"""
+import logging
+import math
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import band, pw
+from ..config import JrystalConfigDict
+from ..crystal import Crystal
+from ..hamiltonian import band_plan
+from ..optim import Adam
+from .energy_all_electrons import calc_with_output as ground_state_calc
+from .energy_all_electrons import create_crystal, create_freq_mask
+
+log = logging.getLogger("jrystal_amd")
+
+
+@dataclass
+class BandStructureOutput:
+  config: JrystalConfigDict
+  crystal: Crystal
+  k_path: np.ndarray           # (nk, 3) 1/bohr
+  eigenvalues: np.ndarray      # (spin, nk, band) Ha
+  params_pw: dict              # the last k-point's parameters
+  ground_state_energy: float
+  density: object = None
+  seconds: float = 0.0
+  evaluations: int = 0
+  save_path: str = ""
+
+
+def k_path_of(config, crystal) -> np.ndarray:
+  """k_path_file (fractional rows) if given, else get_k_path."""
+  B = crystal.reciprocal_vectors
+  if config.k_path_file:
+    frac = np.load(config.k_path_file, allow_pickle=False)
+    frac = np.asarray(frac, np.float64).reshape(-1, 3)
+    return frac @ B
+  return band.get_k_path(crystal.cell_vectors, config.k_path_special_points,
+                         config.num_kpoints, fractional=False)
+
+
+def _adam_steps(plan, params, opt, steps, out):
+  for _ in range(steps):
+    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+    opt.step(params, {"w_re": out["dw_re"], "w_im": out["dw_im"]})
+  return steps
+
+
+def _cg_steps(plan, params, steps):
+  from ..minimize import Minimizer
+  m = Minimizer(plan, params)
+  res = m.run(max_iter=steps)
+  w = m.params()
+  params["w_re"].copy_(w["w_re"])
+  params["w_im"].copy_(w["w_im"])
+  return res.evaluations
+
+
+def calc_with_output(config: JrystalConfigDict, ground_state=None,
+                     ) -> BandStructureOutput:
+  """ground_state: an energy-driver output to take the density from (None:
+  run the energy driver with this config, as the reference does)."""
+  if config.use_pseudopotential:
+    raise NotImplementedError("norm-conserving band structures are outside "
+                              "the all-electron path")
+  if config.optimizer not in ("adam", "cg"):
+    raise NotImplementedError(f"band-structure optimizer {config.optimizer!r}"
+                              ": adam (the reference's) or cg")
+  cr = create_crystal(config)
+  k_path = k_path_of(config, cr)
+  log.info("%d k-points generated.", k_path.shape[0])
+  log.info("===> Starting total energy minimization...")
+  if ground_state is None:
+    ground_state = ground_state_calc(config)
+  rho = torch.as_tensor(ground_state.density, dtype=torch.float64)
+  mask = create_freq_mask(config, cr)
+  num_bands = math.ceil(cr.num_electron / 2) + config.band_structure_empty_bands
+  plan = band_plan(cr.positions, cr.charges, cr.cell_vectors, mask, rho,
+                   k_path[0:1], num_bands, xc=config.xc,
+                   chunk_states=config.chunk_states)
+  params = pw.param_init(config.seed, num_bands, 1, mask,
+                         spin_restricted=config.spin_restricted,
+                         dist=config.param_dist)
+  params = {k: torch.as_tensor(v, dtype=torch.float64).to(plan.device)
+            .contiguous() for k, v in params.items()}
+  if params["w_re"].shape[0] != plan.ns:
+    raise ValueError(f"the ground-state density has {plan.ns} spin "
+                     f"channel(s), the band parameters "
+                     f"{params['w_re'].shape[0]}")
+  log.info("===> Starting band structure calculation...")
+  log.info("Number of bands: %d", num_bands)
+  opt = Adam(params, **dict(config.optimizer_args))
+  out = {}
+  eig = torch.empty((plan.ns, k_path.shape[0], num_bands),
+                    dtype=torch.float64, device=plan.device)
+  evals = 0
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for i in range(k_path.shape[0]):
+    if i > 0:
+      plan.set_kpoints(k_path[i:i + 1])
+    steps = (config.band_structure_epoch if i == 0
+             else config.k_path_fine_tuning_epoch)
+    if config.optimizer == "adam":
+      evals += _adam_steps(plan, params, opt, steps, out)
+    else:
+      evals += _cg_steps(plan, params, steps)
+    plan.evaluate(params["w_re"], params["w_im"], grad=False, out=out)
+    plan.subspace_eigenvalues(out=out.setdefault(
+      "eig", torch.empty((plan.ns, 1, num_bands), dtype=torch.float64,
+                         device=plan.device)))
+    eig[:, i] = out["eig"][:, 0]
+    evals += 1
+    if config.verbose and (i == 0 or (i + 1) % 10 == 0):
+      log.info("k-point %d/%d | trace %.8f Ha", i + 1, k_path.shape[0],
+               float(out["energies"][4]))
+  torch.cuda.synchronize()
+  seconds = time.perf_counter() - t0
+  log.info("Band structure calculation: %.2f seconds", seconds)
+  ev = eig.cpu().numpy()
+  save_file = "".join(cr.symbols) + "_band_structure.npy"
+  if config.save_dir:
+    os.makedirs(config.save_dir, exist_ok=True)
+    save_file = os.path.join(config.save_dir, save_file)
+  np.save(save_file, ev)
+  log.info("Results saved in %s", save_file)
+  return BandStructureOutput(
+    config=config, crystal=cr, k_path=k_path, eigenvalues=ev,
+    params_pw=params, ground_state_energy=ground_state.total_energy,
+    density=rho, seconds=seconds, evaluations=evals, save_path=save_file)
+
+
+def calc(config: JrystalConfigDict) -> np.ndarray:
+  """Eigenvalues (spin, k, band) in Ha, as the reference saves them."""
+  return calc_with_output(config).eigenvalues

@@ -137,7 +137,7 @@ __device__ __forceinline__ double lda_x_deps(double n, double cx) {
 __global__ __launch_bounds__(kRedBlock) void potential_real_kernel(
     const double2* __restrict__ vbox, const double* __restrict__ rho,
     double* __restrict__ v, double* __restrict__ exc_part, int ns, int64_t N,
-    double vol, double cx) {
+    double vol, double cx, int ks) {
   __shared__ double red[kRedBlock];
   const int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
   double exc = 0.0;
@@ -151,13 +151,15 @@ __global__ __launch_bounds__(kRedBlock) void potential_real_kernel(
       v[i] = vhe + w * (4.0 / 3.0) * e;
     } else {
       // Spin-polarised exchange as in xc.py:56-59: eps = (e(2ru)+e(2rd))/2
-      // multiplies the total density.
+      // multiplies the total density.  v_s = d E / d rho_s = eps + rt
+      // d eps / d rho_s; the Kohn-Sham form (ks) of the band driver is
+      // vxc_lda (xc.py:109-124), eps + rho_s d eps / d rho_s.
       const double ru = rho[i], rd = rho[N + i];
       const double e = 0.5 * (lda_x_eps(2.0 * ru, cx) + lda_x_eps(2.0 * rd, cx));
       const double rt = ru + rd;
       exc = w * e * rt;
-      v[i] = vhe + w * (e + rt * lda_x_deps(2.0 * ru, cx));
-      v[N + i] = vhe + w * (e + rt * lda_x_deps(2.0 * rd, cx));
+      v[i] = vhe + w * (e + (ks ? ru : rt) * lda_x_deps(2.0 * ru, cx));
+      v[N + i] = vhe + w * (e + (ks ? rd : rt) * lda_x_deps(2.0 * rd, cx));
     }
   }
   const double t = block_sum(exc, red);
@@ -166,10 +168,10 @@ __global__ __launch_bounds__(kRedBlock) void potential_real_kernel(
 
 int launch_potential_real(const double2* vbox, const double* rho, double* v,
                           double* exc_part, int ns, int64_t N, double vol,
-                          double cx, int* nparts, hipStream_t s) {
+                          double cx, int* nparts, hipStream_t s, bool ks) {
   const int nb = ceil_div(N, kRedBlock);
   hipLaunchKernelGGL(potential_real_kernel, dim3(nb), dim3(kRedBlock), 0, s,
-                     vbox, rho, v, exc_part, ns, N, vol, cx);
+                     vbox, rho, v, exc_part, ns, N, vol, cx, ks ? 1 : 0);
   JRX_LAUNCH_CHECK();
   *nparts = nb;
   return JRX_OK;

@@ -36,10 +36,13 @@ int launch_potential_rec(const double2* rho_rec, double2* vrec,
 // geometry-only factor of the external term, formed once per plan.
 int launch_structure_term(const CellParams& cp, double2* out, hipStream_t s);
 
-// v_s(r) = Re(vbox(r)) + v_xc,s(r) and partials of E_xc (LDA exchange).
+// v_s(r) = Re(vbox(r)) + v_xc,s(r) and partials of E_xc (LDA exchange);
+// ks: the spin-polarised v_xc in the Kohn-Sham form of the band driver
+// (eps + rho_s d eps / d rho_s instead of d E_xc / d rho_s).
 int launch_potential_real(const double2* vbox, const double* rho, double* v,
                           double* exc_part, int ns, int64_t N, double vol,
-                          double cx, int* nparts, hipStream_t s);
+                          double cx, int* nparts, hipStream_t s,
+                          bool ks = false);
 
 // Partials of |G+k|^2 |q|^2 per (state, g-block), band-major q.
 int launch_kinetic(const double2* q, const double* occ, const double* kpts,

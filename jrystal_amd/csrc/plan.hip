@@ -117,6 +117,10 @@ struct jrx_plan {
   int xline_blocks = 0;  // target workgroups of an x-line pass (0 = 4096)
   bool qr_adaptive = true;  // one CholeskyQR pass where W is well conditioned
   bool band_e = false;      // energy+gradient evaluations also form dE/df
+  // Kohn-Sham band mode: evaluations use this potential (the plan's rho/v
+  // layout, ns * N) and form sum f <psi|T + v|psi>; null = ground state
+  double* d_vfix = nullptr;
+  bool fixed = false;
   void* static_ws = nullptr;  // device address of `scratch` on this device
   int xc = JRX_XC_LDA_X;    // JRX_XC_* terms
   bool rowpath = false;     // row-buffer FFT path (z-major rho and v)
@@ -835,8 +839,8 @@ extern "C" int jrx_plan_ritz_rotation(const jrx_plan* p, double* v,
   JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
   char* ws = static_cast<char*>(workspace);
   return launch_ritz(reinterpret_cast<const double2*>(ws + p->o_S), p->d_occ,
-                     p->P, p->nb, reinterpret_cast<double2*>(v),
-                     ws + p->o_ritz, (hipStream_t)stream);
+                     p->P, p->nb, reinterpret_cast<double2*>(v), nullptr,
+                     false, ws + p->o_ritz, (hipStream_t)stream);
 }
 
 extern "C" int jrx_plan_rotate(const jrx_plan* p, const double* v,
@@ -1076,6 +1080,77 @@ static int timing_end(jrx_plan* p, hipStream_t s, int slot, int cls) {
   return JRX_OK;
 }
 
+#define JRX_T(cls, x)                                   \
+  do {                                                  \
+    if ((rc = timing_begin(p, s, &slot, timed))) return rc; \
+    if ((rc = (x))) return rc;                          \
+    if ((rc = timing_end(p, s, slot, cls))) return rc;  \
+  } while (0)
+
+// v = dE/drho of rho (Hartree + external + xc, the vol/N weight included),
+// both in the plan's layout (z-major on the row path, whose box axes are
+// (z, y, x)).  The Hartree/external partials go to the workspace's eh/ee
+// (*nrec of them) and the xc partials to exc (*nexc).  ks: the Kohn-Sham
+// effective potential of the band driver, which differs from dE/drho only
+// in the spin-polarised LDA exchange term.
+static int potential_stage(jrx_plan* p, double* rho, double* v, char* ws,
+                           int* nrec, int* nexc, hipStream_t s, bool timed,
+                           bool ks = false) {
+  int rc, slot;
+  auto cptr = [&](size_t o) { return reinterpret_cast<double2*>(ws + o); };
+  auto dptr = [&](size_t o) { return reinterpret_cast<double*>(ws + o); };
+  double2* box = cptr(p->o_box);
+  double* eh = dptr(p->o_eh);
+  double* ee = dptr(p->o_ee);
+  double* exc = dptr(p->o_exc);
+  const CellParams& cpb = p->rowpath ? p->cpz : p->cp;
+  const FftLen& L0 = p->rowpath ? p->Lz : p->Lx;
+  const FftLen& L2 = p->rowpath ? p->Lx : p->Lz;
+  JRX_T(JRX_K_POTENTIAL, launch_rho_to_complex(rho, box, p->ns, p->N, s));
+  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, false,
+                               1.0, L0, p->Ly, L2, s));
+  double2* gb = p->o_gb ? cptr(p->o_gb) : nullptr;
+  if (gb)  // density gradients of the GGA terms, from fftn(rho)
+    JRX_T(JRX_K_POTENTIAL,
+          xc_stage_gradients(box, gb, cpb, L0, p->Ly, L2, s));
+  JRX_T(JRX_K_POTENTIAL,
+        launch_potential_rec(box, box, eh, ee, cpb, nrec, s, p->d_sterm));
+  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, true,
+                               1.0 / (double)p->N, L0, p->Ly, L2, s));
+  if (p->xc == JRX_XC_LDA_X)
+    JRX_T(JRX_K_POTENTIAL, launch_potential_real(box, rho, v, exc, p->ns,
+                                                 p->N, p->vol, p->cx, nexc,
+                                                 s, ks));
+  else
+    JRX_T(JRX_K_POTENTIAL,
+          xc_stage_potential(box, rho, v, gb, exc, nexc, cpb, p->xc, L0,
+                             p->Ly, L2, s));
+  return JRX_OK;
+}
+
+// energies = {sum f tb / 2, sum f pot / vol, 0, 0, their sum}: the band
+// energies' trace sum f <psi|T + v|psi> of the Kohn-Sham band mode.
+__global__ __launch_bounds__(256) void band_trace_kernel(
+    const double* __restrict__ tb, const double* __restrict__ pot,
+    const double* __restrict__ occ, int64_t S, double inv_vol,
+    double* __restrict__ energies) {
+  __shared__ double red[4];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < S; i += 256) {
+    a += occ[i] * tb[i];
+    b += occ[i] * pot[i];
+  }
+  const double k = 0.5 * block_sum(a, red);
+  const double u = inv_vol * block_sum(b, red);
+  if (threadIdx.x == 0) {
+    energies[0] = k;
+    energies[1] = u;
+    energies[2] = 0.0;
+    energies[3] = 0.0;
+    energies[4] = k + u;
+  }
+}
+
 // One evaluation into the given workspace; `timed` allows the per-kernel
 // events of jrx_plan_set_timing (their first use creates HIP events).
 static int energy_grad_impl(jrx_plan* p, const double* w_re,
@@ -1100,7 +1175,6 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   double* rho = dptr(p->o_rho);
   double* rpart = dptr(p->o_rpart);
   double* v = dptr(p->o_v);
-  double2* box = cptr(p->o_box);
   double2* Sm = cptr(p->o_S);
   double2* R1 = cptr(p->o_R1);
   double2* R2 = cptr(p->o_R2);
@@ -1117,12 +1191,9 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   const int64_t P = p->P, S = p->S;
   int rc;
   int slot;
-#define JRX_T(cls, x)                                   \
-  do {                                                  \
-    if ((rc = timing_begin(p, s, &slot, timed))) return rc; \
-    if ((rc = (x))) return rc;                          \
-    if ((rc = timing_end(p, s, slot, cls))) return rc;  \
-  } while (0)
+  const bool fixed = p->fixed;
+  JRX_REQUIRE(!(fixed && rho_out),
+              "Kohn-Sham band mode forms no density (rho_out must be NULL)");
   // --- QR (CholeskyQR2; adaptive: the second pass only for the k-points
   // whose first Gram matrix is far from diagonal) --------------------------
   int* one_pass = p->qr_adaptive
@@ -1143,65 +1214,50 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   const int64_t plane_elems =
       p->rowpath ? (int64_t)p->nz * p->geo.rstride
                  : (int64_t)p->geo.nxa * p->geo.ps;
-  for (int sp = 0; sp < p->ns; ++sp) {
-    JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
-    const int64_t step = p->keep_planes ? SS : p->chunk;
-    for (int64_t s0 = 0; s0 < SS; s0 += step) {
-      const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
-      const int64_t st0 = sp * SS + s0;
-      double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
-      if (p->rowpath) {
-        JRX_T(JRX_K_PLANE_IFFT,
-              launch_row_ifft(q, Tc, st0, nst, p->geo, p->Lx.tw, s));
-        JRX_T(JRX_K_XLINE_DENSITY,
-              launch_zplane_density(Tc, rpart, zplane_groups(p->geo, nst),
-                                    p->d_occ, st0, nst, 1.0 / p->vol,
-                                    p->geo, p->Lx.tw, s));
-      } else if (reg64) {
-        JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst, p->geo,
-                                                    p->Lx.tw, s));
-        JRX_T(JRX_K_XLINE_DENSITY,
-              launch_xline_density64(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
-                                     p->d_occ, st0, nst, 1.0 / p->vol,
-                                     p->geo, p->Lx.tw, s));
-      } else {
-        JRX_T(JRX_K_PLANE_IFFT,
-              launch_plane_ifft(q, Tc, st0, nst, p->geo, p->Ly, p->Lz, s));
-        JRX_T(JRX_K_XLINE_DENSITY,
-              launch_xline_density(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
-                                   p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
-                                   p->Lx, s));
-      }
-    }
-    JRX_T(JRX_K_XLINE_DENSITY,
-          launch_reduce_groups(rpart, rho + sp * p->N, p->groups, p->N, s));
-  }
-  // --- reciprocal energies and potential ----------------------------------
-  // (on the row path rho and v are z-major: the box axes are (z, y, x))
-  const CellParams& cpb = p->rowpath ? p->cpz : p->cp;
-  const FftLen& L0 = p->rowpath ? p->Lz : p->Lx;
-  const FftLen& L2 = p->rowpath ? p->Lx : p->Lz;
-  JRX_T(JRX_K_POTENTIAL, launch_rho_to_complex(rho, box, p->ns, p->N, s));
-  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, false,
-                               1.0, L0, p->Ly, L2, s));
   int nrec = 0, nexc = 0, nkin = 0;
-  double2* gb = p->o_gb ? cptr(p->o_gb) : nullptr;
-  if (gb)  // density gradients of the GGA terms, from fftn(rho)
-    JRX_T(JRX_K_POTENTIAL,
-          xc_stage_gradients(box, gb, cpb, L0, p->Ly, L2, s));
-  JRX_T(JRX_K_POTENTIAL,
-        launch_potential_rec(box, box, eh, ee, cpb, &nrec, s, p->d_sterm));
-  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, true,
-                               1.0 / (double)p->N, L0, p->Ly, L2, s));
-  if (p->xc == JRX_XC_LDA_X)
-    JRX_T(JRX_K_POTENTIAL, launch_potential_real(box, rho, v, exc, p->ns,
-                                                 p->N, p->vol, p->cx, &nexc,
-                                                 s));
-  else
-    JRX_T(JRX_K_POTENTIAL,
-          xc_stage_potential(box, rho, v, gb, exc, &nexc, cpb, p->xc, L0,
-                             p->Ly, L2, s));
-  if (dw_re) {
+  if (!fixed) {
+    for (int sp = 0; sp < p->ns; ++sp) {
+      JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
+      const int64_t step = p->keep_planes ? SS : p->chunk;
+      for (int64_t s0 = 0; s0 < SS; s0 += step) {
+        const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
+        const int64_t st0 = sp * SS + s0;
+        double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
+        if (p->rowpath) {
+          JRX_T(JRX_K_PLANE_IFFT,
+                launch_row_ifft(q, Tc, st0, nst, p->geo, p->Lx.tw, s));
+          JRX_T(JRX_K_XLINE_DENSITY,
+                launch_zplane_density(Tc, rpart, zplane_groups(p->geo, nst),
+                                      p->d_occ, st0, nst, 1.0 / p->vol,
+                                      p->geo, p->Lx.tw, s));
+        } else if (reg64) {
+          JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst, p->geo,
+                                                      p->Lx.tw, s));
+          JRX_T(JRX_K_XLINE_DENSITY,
+                launch_xline_density64(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
+                                       p->d_occ, st0, nst, 1.0 / p->vol,
+                                       p->geo, p->Lx.tw, s));
+        } else {
+          JRX_T(JRX_K_PLANE_IFFT,
+                launch_plane_ifft(q, Tc, st0, nst, p->geo, p->Ly, p->Lz, s));
+          JRX_T(JRX_K_XLINE_DENSITY,
+                launch_xline_density(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
+                                     p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
+                                     p->Lx, s));
+        }
+      }
+      JRX_T(JRX_K_XLINE_DENSITY,
+            launch_reduce_groups(rpart, rho + sp * p->N, p->groups, p->N, s));
+    }
+    // --- reciprocal energies and potential --------------------------------
+    if ((rc = potential_stage(p, rho, v, ws, &nrec, &nexc, s, timed)))
+      return rc;
+  }
+  const double* vv = fixed ? p->d_vfix : v;
+  // the density pass left every state's planes in T (keep_planes)
+  const bool planes_ready = p->keep_planes && !fixed;
+  const bool pot_e = p->band_e || fixed;
+  if (dw_re || fixed) {
     // --- adjoint sweep: qbar = f (2/vol FFT(v psi) + |G+k|^2 q) -------------
     GatherParams gp;
     gp.occ = p->d_occ;
@@ -1210,7 +1266,7 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
     gp.nk = p->nk;
     gp.nb = nb;
     gp.two_over_vol = 2.0 / p->vol;
-    gp.pot_partial = p->band_e ? dptr(p->o_bepart) : nullptr;
+    gp.pot_partial = pot_e ? dptr(p->o_bepart) : nullptr;
     gp.g_m = p->d_g_m;
     const int64_t step = p->keep_planes ? SS : p->chunk;
     for (int sp = 0; sp < p->ns; ++sp) {
@@ -1219,33 +1275,33 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
         const int64_t st0 = sp * SS + s0;
         double2* Tc = p->keep_planes ? T + st0 * plane_elems : T;
         if (p->rowpath) {
-          if (!p->keep_planes)
+          if (!planes_ready)
             JRX_T(JRX_K_PLANE_IFFT,
                   launch_row_ifft(q, Tc, st0, nst, p->geo, p->Lx.tw, s));
           JRX_T(JRX_K_XLINE_VPSI,
-                launch_zplane_vpsi(Tc, v + sp * p->N,
+                launch_zplane_vpsi(Tc, vv + sp * p->N,
                                    zplane_groups(p->geo, nst), nst, p->geo,
                                    p->Lx.tw, s));
           JRX_T(JRX_K_PLANE_FFT_GATHER,
                 launch_row_fft_gather(Tc, q, qbar, kin, st0, nst, p->geo,
                                       p->Lx.tw, gp, s));
         } else if (reg64) {
-          if (!p->keep_planes)
+          if (!planes_ready)
             JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst,
                                                         p->geo, p->Lx.tw, s));
           JRX_T(JRX_K_XLINE_VPSI,
-                launch_xline_vpsi64(Tc, v + sp * p->N,
+                launch_xline_vpsi64(Tc, vv + sp * p->N,
                                     xline_groups(p->geo, nst, p->xline_blocks), nst, p->geo,
                                     p->Lx.tw, s));
           JRX_T(JRX_K_PLANE_FFT_GATHER,
                 launch_plane_fft_gather64(Tc, q, qbar, kin, st0, nst, p->geo,
                                           p->Lx.tw, gp, s));
         } else {
-          if (!p->keep_planes)
+          if (!planes_ready)
             JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft(q, Tc, st0, nst, p->geo,
                                                       p->Ly, p->Lz, s));
           JRX_T(JRX_K_XLINE_VPSI,
-                launch_xline_vpsi(Tc, v + sp * p->N, xline_groups(p->geo, nst, p->xline_blocks),
+                launch_xline_vpsi(Tc, vv + sp * p->N, xline_groups(p->geo, nst, p->xline_blocks),
                                   nst, p->geo, p->Lx, s));
           JRX_T(JRX_K_PLANE_FFT_GATHER,
                 launch_plane_fft_gather(Tc, q, qbar, kin, st0, nst, p->geo,
@@ -1255,21 +1311,30 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
     }
     const int nper = p->rowpath ? p->row_chunks : p->geo.nxa;
     JRX_T(JRX_K_OTHER, launch_state_sums(kin, nper, S, tb, s));
-    if (p->band_e)
+    if (pot_e)
       JRX_T(JRX_K_OTHER, launch_state_sums(dptr(p->o_bepart), nper, S,
                                            dptr(p->o_pot), s));
-    // --- QR adjoint ----------------------------------------------------------
+    // --- QR adjoint (Q^H Qbar alone without a gradient: the subspace
+    // matrix of jrx_plan_subspace_eigenvalues) --------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
     JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsq.nchunk, nb, s));
-    JRX_T(JRX_K_QR_SMALL, launch_qr_bwd_small(Sm, Ri, Am, Bm, P, nb, s));
-    JRX_T(JRX_K_QR_BWD_APPLY, launch_apply_bwd(qbar, q, Am, Bm, dw_re, dw_im, P, ng, nb, s));
+    if (dw_re) {
+      JRX_T(JRX_K_QR_SMALL, launch_qr_bwd_small(Sm, Ri, Am, Bm, P, nb, s));
+      JRX_T(JRX_K_QR_BWD_APPLY, launch_apply_bwd(qbar, q, Am, Bm, dw_re, dw_im, P, ng, nb, s));
+    }
   } else {
     JRX_T(JRX_K_OTHER, launch_kinetic(q, p->d_occ, p->d_kpts, p->d_g_m, p->cp, S, ng,
                            p->nk, nb, kin, &nkin, s));
     JRX_T(JRX_K_OTHER, launch_state_sums(kin, (int)(nkin / S), S, tb, s));
   }
-  JRX_T(JRX_K_OTHER, launch_finalize_energies(tb, p->d_occ, S, eh, ee, nrec,
-                                              exc, nexc, energies, s));
+  if (fixed) {
+    hipLaunchKernelGGL(band_trace_kernel, dim3(1), dim3(256), 0, s, tb,
+                       dptr(p->o_pot), p->d_occ, S, 1.0 / p->vol, energies);
+    JRX_LAUNCH_CHECK();
+  } else {
+    JRX_T(JRX_K_OTHER, launch_finalize_energies(tb, p->d_occ, S, eh, ee, nrec,
+                                                exc, nexc, energies, s));
+  }
   if (rho_out && p->rowpath) {
     if ((rc = launch_zmajor_to_xmajor(rho, rho_out, p->ns, p->nx, s)))
       return rc;
@@ -1310,6 +1375,73 @@ int jrx_energy_grad_static(jrx_plan* p, const double* w_re,
   return energy_grad_impl(p, w_re, w_im, energies, dw_re, dw_im, rho_out,
                           q_out, p->static_ws, JRX_STATIC_WORKSPACE_BYTES,
                           stream, false);
+}
+
+int jrx_plan_set_fixed_density(jrx_plan* p, const double* rho,
+                               void* workspace, size_t workspace_bytes,
+                               void* stream) {
+  JRX_REQUIRE(p, "null plan");
+  if (!rho) {
+    p->fixed = false;
+    return JRX_OK;
+  }
+  JRX_REQUIRE(workspace && workspace_bytes >= p->total,
+              "workspace too small (see jrx_plan_workspace_bytes)");
+  JRX_REQUIRE(((uintptr_t)workspace & 255) == 0,
+              "workspace must be 256-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = (size_t)p->ns * p->N * sizeof(double);
+  if (!p->d_vfix) {
+    double* d = nullptr;
+    JRX_CHECK_HIP(hipMalloc(&d, bytes));
+    p->allocs.push_back(d);
+    p->d_vfix = d;
+  }
+  char* ws = static_cast<char*>(workspace);
+  double* r = reinterpret_cast<double*>(ws + p->o_rho);
+  int rc;
+  if (p->rowpath) {
+    // (x, y, z) -> the row path's (z, y, x): on its cubic box the axis swap
+    // is its own inverse
+    if ((rc = launch_zmajor_to_xmajor(rho, r, p->ns, p->nx, s))) return rc;
+  } else {
+    JRX_CHECK_HIP(hipMemcpyAsync(r, rho, bytes, hipMemcpyDeviceToDevice, s));
+  }
+  int nrec = 0, nexc = 0;
+  if ((rc = potential_stage(p, r, p->d_vfix, ws, &nrec, &nexc, s, false,
+                            true)))
+    return rc;
+  p->fixed = true;
+  return JRX_OK;
+}
+
+int jrx_plan_set_kpoints(jrx_plan* p, const double* kpts, void* stream) {
+  JRX_REQUIRE(p && kpts, "null argument");
+  JRX_CHECK_HIP(hipMemcpyAsync(p->d_kpts, kpts, (size_t)p->nk * 3 * sizeof(double),
+                               hipMemcpyDefault, (hipStream_t)stream));
+  return JRX_OK;
+}
+
+int jrx_plan_subspace_eigenvalues(const jrx_plan* p, double* evals,
+                                  double* vecs, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && evals && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  char* ws = static_cast<char*>(workspace);
+  return launch_ritz(reinterpret_cast<const double2*>(ws + p->o_S), p->d_occ,
+                     p->P, p->nb, reinterpret_cast<double2*>(vecs), evals,
+                     true, ws + p->o_ritz, (hipStream_t)stream);
+}
+
+int jrx_plan_subspace_hamiltonian(const jrx_plan* p, double* h,
+                                  const void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  JRX_REQUIRE(p && h && workspace, "null argument");
+  JRX_REQUIRE(workspace_bytes >= p->total, "workspace too small");
+  const char* ws = static_cast<const char*>(workspace);
+  return launch_subspace_h(reinterpret_cast<const double2*>(ws + p->o_S),
+                           p->d_occ, p->P, p->nb, reinterpret_cast<double2*>(h),
+                           (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -83,10 +83,16 @@ int launch_apply_bwd(const double2* qbar, const double2* q, const double2* A,
 
 // Rayleigh-Ritz rotation (ritz.hip): V[p] = eigenvectors, ascending, of
 // the Hermitised (S[p] / (2 f)) on each run of >= 2 consecutive bands with
-// equal occupation f > 0, identity elsewhere.  scratch:
-// ritz_scratch_bytes(P, nb), 256-B aligned.
+// equal occupation f > 0, identity elsewhere; whole: one run of all nb
+// bands (every f > 0).  V and evals (the ascending eigenvalues at each
+// run's bands) are optional.  scratch: ritz_scratch_bytes(P, nb), 256-B
+// aligned.
 size_t ritz_scratch_bytes(int64_t P, int nb);
 int launch_ritz(const double2* S, const double* occ, int64_t P, int nb,
-                double2* V, void* scratch, hipStream_t s);
+                double2* V, double* evals, bool whole, void* scratch,
+                hipStream_t s);
+// H[p]_ij = q_i^H H q_j = (S_ij / (2 f_j) + conj(S_ji) / (2 f_i)) / 2.
+int launch_subspace_h(const double2* S, const double* occ, int64_t P, int nb,
+                      double2* H, hipStream_t s);
 
 }  // namespace jrx

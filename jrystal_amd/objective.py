@@ -131,6 +131,7 @@ class EnergyPlan:
     self._ws = torch.empty(self.workspace_bytes + 256, dtype=torch.uint8,
                            device=self.device)
     self._ws_ptr = (self._ws.data_ptr() + 255) // 256 * 256
+    self.band_mode = False
 
   @classmethod
   def from_workload(cls, wl, chunk_states=0, device=None, **options):
@@ -328,6 +329,71 @@ class EnergyPlan:
         self._plan, _lib.ptr(out), self._ws_ptr, self.workspace_bytes,
         _lib.stream_ptr(self.device)))
     return out
+
+  def set_fixed_density(self, density):
+    """Kohn-Sham band mode: later evaluations use the effective potential
+    v_eff = V_H + V_ext + v_xc of ``density`` ((ns,nx,ny,nz), the
+    ground-state density; Kohn-Sham form of potential.py:203-279) and
+    return E = sum_{s,k,b} f <psi|T + v_eff|psi> (energies = kinetic
+    part, potential part, 0, 0, E) and its gradient; density=None returns
+    to the ground-state objective.  Graphs captured before a switch replay
+    the old objective."""
+    ptr = None
+    if density is not None:
+      rho = torch.as_tensor(density, dtype=torch.float64)
+      shape = (self.ns,) + tuple(self.grid_sizes)
+      if tuple(rho.shape) != shape:
+        raise ValueError(f"density must have shape {shape}, got "
+                         f"{tuple(rho.shape)}")
+      rho = rho.to(self.device).contiguous()
+      self._fixed_rho = rho  # alive until the potential has been formed
+      ptr = _lib.ptr(rho)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_set_fixed_density(
+        self._plan, ptr, self._ws_ptr, self.workspace_bytes,
+        _lib.stream_ptr(self.device)))
+    self.band_mode = density is not None
+
+  def set_kpoints(self, kpts):
+    """Replace the k-points (nk,3) [1/bohr] of later evaluations."""
+    k = torch.as_tensor(np.asarray(kpts, np.float64).reshape(-1, 3))
+    if k.shape[0] != self.nk:
+      raise ValueError(f"expected {self.nk} k-points, got {k.shape[0]}")
+    k = k.to(self.device).contiguous()
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_set_kpoints(
+        self._plan, _lib.ptr(k), _lib.stream_ptr(self.device)))
+    self._kpts_dev = k  # alive until the copy has run
+
+  def subspace_hamiltonian(self, out=None):
+    """q_i^H H q_j of the last evaluation (with a gradient, or any
+    evaluation in band mode), (ns,nk,nb,nb) complex128 on the device; NaN
+    where an occupation is <= 0."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb, self.nb),
+                        dtype=torch.complex128, device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_subspace_hamiltonian(
+        self._plan, _lib.ptr(out), self._ws_ptr, self.workspace_bytes,
+        _lib.stream_ptr(self.device)))
+    return out
+
+  def subspace_eigenvalues(self, vectors: bool = False, out=None):
+    """Ascending eigenvalues (ns,nk,nb) of subspace_hamiltonian(), and with
+    ``vectors`` its eigenvectors (ns,nk,nb,nb) complex128 (columns); cyclic
+    Jacobi on the device."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb), dtype=torch.float64,
+                        device=self.device)
+    vec = None
+    if vectors:
+      vec = torch.empty((self.ns, self.nk, self.nb, self.nb),
+                        dtype=torch.complex128, device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_subspace_eigenvalues(
+        self._plan, _lib.ptr(out), _lib.ptr(vec), self._ws_ptr,
+        self.workspace_bytes, _lib.stream_ptr(self.device)))
+    return (out, vec) if vectors else out
 
   def rotate(self, v, out_re=None, out_im=None):
     """Q V of the last evaluation in the (s,k,g,b) parameter layout;

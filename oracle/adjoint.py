@@ -50,17 +50,14 @@ class Problem:
     return self.g_vector_grid[self.mask]
 
 
-def potential_real(rho, prob: Problem):
-  """v_s(r) = dE/drho_s(r) for E = E_H + E_ext + E_xc (vol/N included).
-
-  Returns (v (s,x,y,z), rho_rec (s,x,y,z), energies dict)."""
+def hartree_external_potential(rho_rec, prob: Problem):
+  """V_H + V_ext on the grid with the vol/N weight, from rho_rec = fftn(rho)
+  (s,x,y,z): V_H = ifftn(4 pi rho_tot(G) / |G|^2), G = 0 dropped
+  (potential.py:24-77 with kohn_sham), V_ext = ifftn of
+  -4 pi sum_a Z_a e^{-iG.R_a} / (|G|^2 + 1e-10) (potential.py:121-166)."""
   g = prob.g_vector_grid
   vol = prob.vol
-  n = np.prod(rho.shape[-3:])
-  rho_rec = sfft.fftn(rho, axes=(-3, -2, -1), workers=_pw._WORKERS)
-  e_har = _en.hartree(rho_rec, g, vol)
-  e_ext = _en.external(rho_rec, prob.positions, prob.charges, g, vol)
-  e_xc = _en.xc_energy(rho, g, vol, prob.xc)
+  n = np.prod(rho_rec.shape[-3:])
   rho_tot_rec = np.sum(rho_rec, axis=0)
   g2 = np.sum(g**2, axis=-1)
   g2h = g2.copy()
@@ -72,6 +69,21 @@ def potential_real(rho, prob: Problem):
   ve_rec = s_conj / (g2 + 1e-10)
   ve_rec[0, 0, 0] = 0.0
   v_e = -4 * np.pi * np.real(sfft.ifftn(ve_rec, workers=_pw._WORKERS))
+  return v_h + v_e
+
+
+def potential_real(rho, prob: Problem):
+  """v_s(r) = dE/drho_s(r) for E = E_H + E_ext + E_xc (vol/N included).
+
+  Returns (v (s,x,y,z), rho_rec (s,x,y,z), energies dict)."""
+  g = prob.g_vector_grid
+  vol = prob.vol
+  n = np.prod(rho.shape[-3:])
+  rho_rec = sfft.fftn(rho, axes=(-3, -2, -1), workers=_pw._WORKERS)
+  e_har = _en.hartree(rho_rec, g, vol)
+  e_ext = _en.external(rho_rec, prob.positions, prob.charges, g, vol)
+  e_xc = _en.xc_energy(rho, g, vol, prob.xc)
+  v_he = hartree_external_potential(rho_rec, prob)
   if prob.xc != "lda_x":
     from . import xc as _xc
     v_xc = _xc.xc_potential(rho, g, vol, prob.xc)
@@ -87,7 +99,7 @@ def potential_real(rho, prob: Problem):
       de = np.where(n2 > _en.LDA_DENS_THRESHOLD,
                     -_en.CX / 3.0 * np.cbrt(np.maximum(n2, 1e-300))**-2, 0.0)
       v_xc[s] = (vol / n) * (eps + rho_tot * de)
-  v = v_h[None] + v_e[None] + v_xc
+  v = v_he[None] + v_xc
   return v, rho_rec, {"hartree": e_har, "external": e_ext, "xc": e_xc}
 
 

@@ -104,11 +104,33 @@ def xc_density(density_grid, xc_type="lda_x"):
   return lda_x_eps(density_grid[0])
 
 
+def vxc_lda_x(density_grid):
+  """xc.py:109-124 for LDA exchange (xc_density with kohn_sham, xc.py:
+  247-250): eps + rho_s d eps / d rho_s per spin channel, (s,x,y,z); for
+  one channel d(rho eps)/d rho = 4/3 eps."""
+  if density_grid.shape[0] == 1:
+    return (4.0 / 3.0) * lda_x_eps(density_grid[0])[None]
+  eps = xc_density(density_grid)
+  out = np.empty_like(density_grid)
+  for s in range(2):
+    n2 = 2 * density_grid[s]
+    de = np.where(n2 > LDA_DENS_THRESHOLD,
+                  -CX / 3.0 * np.cbrt(np.maximum(n2, 1e-300))**-2, 0.0)
+    out[s] = eps + density_grid[s] * de
+  return out
+
+
 def xc_energy(density_grid, g_vector_grid, vol, xc_type="lda_x",
               kohn_sham=False):
-  """energy.py:185-211: sum_r eps(rho) * rho_s summed over spin, * vol/N."""
+  """energy.py:185-211: sum_r eps(rho) * rho_s summed over spin, * vol/N;
+  with kohn_sham (LDA exchange) the potential vxc_lda takes the place of
+  eps, channel by channel."""
   assert density_grid.ndim == 4
   n = np.prod(density_grid.shape[-3:])
+  if kohn_sham:
+    if xc_type != "lda_x":
+      raise NotImplementedError("kohn_sham xc energy: lda_x only")
+    return safe_real(np.sum(vxc_lda_x(density_grid) * density_grid)) * vol / n
   if xc_type != "lda_x":
     from . import xc as _xc
     return _xc.xc_energy(density_grid, g_vector_grid, vol, xc_type)
