@@ -20,7 +20,8 @@ path runs on one GPU), --steps K, --warmup W, --no-cpu-baseline,
 --no-batch, --batch-steps K (the uneven-structure batch section),
 --traffic-probe (the workload of the rocprofv3 counter runs),
 --no-large-cell, --large-cell-steps K, --large-cell-probe (the 128^3 /
-128-band section and its rocprofv3 workload), --no-small-cell.
+128-band section and its rocprofv3 workload), --no-small-cell,
+--no-band-structure.
 
 A third section, "batch", times Adam steps (energy + gradient + update) of
 16 uneven structures run as one batch: eager, over several streams, and as
@@ -33,6 +34,10 @@ graph.
 A fourth, "large_cell", times evaluations of the 64-atom Si cell at the
 Gamma point (128^3 box, 128 bands) with the FP64 MFMA rate of its QR
 passes.
+
+A sixth, "band_structure", runs the band driver on the reference's Si band
+example (60-point path, 28 bands, 64^3): its whole Adam schedule, and the
+path with CG at each k-point.
 """
 import argparse
 import json
@@ -338,6 +343,71 @@ def small_cell_bench(steps, warmup):
   return res
 
 
+def band_structure_bench(adam_first=10000, adam_rest=400, num_kpoints=60):
+  """The reference's all-electron Si band example (tests/all_electron/si/
+  config.yaml: geometry/si.xyz, 64^3, spherical 100 Ha, 28 bands, the
+  60-point path LGXL, Adam with lr 0.001, 10000 steps at the first k-point
+  and 400 at each of the others) through the band driver, from a synthetic
+  fixed density (that of 14 seeded orbitals at Gamma; the cost of a band
+  step does not depend on the density's values): the whole Adam schedule,
+  and the same path with preconditioned CG at each k-point."""
+  import tempfile
+  from dataclasses import dataclass
+  import importlib
+  import numpy as np
+  import torch
+  from jrystal_amd import config as C
+  from jrystal_amd import pw
+  from jrystal_amd.objective import EnergyPlan
+  BD = importlib.import_module("jrystal_amd.calc.band_all_electrons")
+
+  @dataclass
+  class _GS:
+    density: object
+    total_energy: float = 0.0
+
+  tmp = tempfile.mkdtemp(prefix="jrx_band_")
+  cfg = C.get_config()
+  cfg.update(crystal="si", grid_sizes=64, cutoff_energy=100.0,
+             freq_mask_method="spherical", empty_bands=14,
+             band_structure_empty_bands=14, xc="lda_x",
+             k_path_special_points="LGXL", num_kpoints=num_kpoints,
+             band_structure_epoch=adam_first,
+             k_path_fine_tuning_epoch=adam_rest, optimizer="adam",
+             optimizer_args={"learning_rate": 0.001, "b1": 0.9, "b2": 0.99},
+             seed=123, save_dir=tmp, verbose=False)
+  cr = BD.create_crystal(cfg)
+  mask = BD.create_freq_mask(cfg, cr)
+  gsp = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, mask,
+                   np.zeros((1, 3)), 2.0 * np.ones((1, 1, 14)))
+  p0 = pw.param_init(0, 14, 1, mask)
+  rho = gsp.evaluate(p0["w_re"], p0["w_im"], grad=False,
+                     density=True)["density"].clone()
+  gsp.close()
+  print("bench: band structure (Adam schedule)", file=sys.stderr, flush=True)
+  adam = BD.calc_with_output(cfg, ground_state=_GS(rho))
+  res = {"workload": "si-fcc-64^3-100Ha-28b-LGXL-60k",
+         "num_g": int(np.sum(np.asarray(mask))), "num_bands": 28,
+         "num_kpoints": num_kpoints, "density": "synthetic (14 seeded "
+         "orbitals at Gamma)",
+         "adam": {"steps": adam_first + (num_kpoints - 1) * adam_rest,
+                  "evaluations": adam.evaluations,
+                  "seconds": adam.seconds,
+                  "ms_per_step": 1e3 * adam.seconds / adam.evaluations}}
+  cfg.update(optimizer="cg", band_structure_epoch=400,
+             k_path_fine_tuning_epoch=200)
+  print("bench: band structure (CG)", file=sys.stderr, flush=True)
+  cg = BD.calc_with_output(cfg, ground_state=_GS(rho))
+  res["cg"] = {"evaluations": cg.evaluations, "seconds": cg.seconds,
+               "max_iter": [400, 200]}
+  d = np.abs(adam.eigenvalues - cg.eigenvalues)
+  res["max_abs_eig_diff_adam_vs_cg_Ha"] = float(np.max(d))
+  res["trace_sum_lowest_adam_minus_cg_Ha"] = float(
+    np.max(np.sum(adam.eigenvalues - cg.eigenvalues, axis=-1)))
+  torch.cuda.empty_cache()
+  return res
+
+
 def large_cell_bench(steps, warmup):
   """Energy + gradient evaluations of the 64-atom Si cell at the Gamma
   point (128^3 box, 128 bands, 48 Ha, 137059 coefficients per band), with
@@ -541,6 +611,7 @@ def main():
   ap.add_argument("--traffic-probe", action="store_true",
                   help="run the rocprofv3 counter workload and exit")
   ap.add_argument("--no-small-cell", action="store_true")
+  ap.add_argument("--no-band-structure", action="store_true")
   ap.add_argument("--no-large-cell", action="store_true")
   ap.add_argument("--large-cell-steps", type=int, default=5)
   ap.add_argument("--large-cell-probe", action="store_true",
@@ -655,6 +726,10 @@ def main():
   if not args.no_large_cell:
     large = large_cell_bench(args.large_cell_steps, 1)
 
+  bands = None
+  if not args.no_band_structure:
+    bands = band_structure_bench()
+
   conv = None
   if not args.no_convergence:
     conv, conv_ctx = convergence(args.conv_seconds)
@@ -698,6 +773,7 @@ def main():
     "batch": batch,
     "small_cell": small,
     "large_cell": large,
+    "band_structure": bands,
   }
   print(json.dumps(line))
 

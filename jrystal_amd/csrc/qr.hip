@@ -807,10 +807,20 @@ __global__ __launch_bounds__(kQrBlock) void reduce_partials_kernel(
   const int64_t p = blockIdx.y;
   const int ol = threadIdx.x % kRedOut, grp = threadIdx.x / kRedOut;
   const int o = blockIdx.x * kRedOut + ol;
-  double2 a = czero();
-  if (o < nout)
-    for (int c = grp; c < nchunk; c += kRedGroups)
-      a = cadd(a, partial[(p * nchunk + c) * nout + o]);
+  // four independent sums in flight, added in a fixed order
+  double2 a = czero(), b = czero(), c2 = czero(), d = czero();
+  if (o < nout) {
+    const double2* src = partial + p * nchunk * nout + o;
+    int c = grp;
+    for (; c + 3 * kRedGroups < nchunk; c += 4 * kRedGroups) {
+      a = cadd(a, src[(int64_t)c * nout]);
+      b = cadd(b, src[(int64_t)(c + kRedGroups) * nout]);
+      c2 = cadd(c2, src[(int64_t)(c + 2 * kRedGroups) * nout]);
+      d = cadd(d, src[(int64_t)(c + 3 * kRedGroups) * nout]);
+    }
+    for (; c < nchunk; c += kRedGroups) a = cadd(a, src[(int64_t)c * nout]);
+  }
+  a = cadd(cadd(a, b), cadd(c2, d));
   acc[grp][ol] = a;
   __syncthreads();
   if (grp == 0 && o < nout) {
@@ -829,11 +839,27 @@ int launch_reduce_partials(const double2* partial, double2* out, int64_t P,
   return JRX_OK;
 }
 
-// One block per problem; A and X live in LDS ([nb][nb+1] each).
+// One block per problem: the Cholesky factor of S = R^H R (R upper) and
+// rinv = X = R^{-1}, in one sweep of nb + 3 steps with one barrier each.
+// Step s:
+//  (a) updates the trailing rows with the unscaled pivot row s,
+//      A[r][c] -= conj(A[s][r]) A[s][c] / A[s][s];
+//  (b) scales row s - 1 into R (R[j][j] = sqrt of the pivot, NaN for a
+//      non-positive one);
+//  (c) adds row x = s - 2 of R into the partial sums of X R = I: with
+//      X[i][c] holding P[i][c] = sum_{k<x} X[i][k] R[k][c] for c >= x,
+//      X[i][c] += X_ix R[x][c] (c > x), X_ix = -P[i][x] / R[x][x]
+//      (1 / R[x][x] for i = x);
+//  (d) writes column y = s - 3 of X from its complete partial sums.
+// (a) writes rows > s of A, (b) row s - 1, (c) and (d) read rows <= s - 2;
+// (c) writes columns > s - 2 of X and (d) column s - 3, so the parts of a
+// step touch disjoint data.  A (then R) and X live in LDS ([nb][nb+1]
+// each).
 __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
     const double2* __restrict__ S, double2* __restrict__ rinv, int nb,
     int* __restrict__ flag_out, const int* __restrict__ flag_in) {
   extern __shared__ double2 sm[];
+  __shared__ double dg[kNarrowBands];  // R[j][j]
   const int ld = nb + 1;
   double2* A = sm;
   double2* X = sm + nb * ld;
@@ -862,42 +888,167 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv_kernel(
         a += (v.x * v.x + v.y * v.y) / (A[i * ld + i].x * A[j * ld + j].x);
       }
     }
-    // X is free until the back substitution
+    // X is free until the sweep
     const double d2 = block_sum(a, reinterpret_cast<double*>(X));
     if (tid == 0)  // NaN or a non-positive diagonal compares false
       flag_out[p] = (d2 < kQrOnePass * kQrOnePass) ? 1 : 0;
   }
-  for (int j = 0; j < nb; ++j) {
-    if (tid == 0) {
-      const double d = A[j * ld + j].x;
-      // NaN for a non-positive pivot: sqrt of a negative number.
-      A[j * ld + j] = make_double2(d > 0.0 ? sqrt(d) : sqrt(-1.0 - d * 0.0), 0.0);
-    }
-    __syncthreads();
-    const double d = A[j * ld + j].x;
-    for (int c = j + 1 + tid; c < nb; c += kQrBlock)
-      A[j * ld + c] = cscale(A[j * ld + c], 1.0 / d);
-    __syncthreads();
-    const int m = nb - j - 1;
-    for (int e = tid; e < m * m; e += kQrBlock) {
-      const int r = j + 1 + e / m, c = j + 1 + e % m;
-      if (c >= r)
-        A[r * ld + c] = csub(A[r * ld + c], cmulc(A[j * ld + r], A[j * ld + c]));
-    }
-    __syncthreads();
-  }
-  // X = R^{-1}, column c by thread c: back substitution.
-  for (int c = tid; c < nb; c += kQrBlock) {
-    for (int i = c; i >= 0; --i) {
-      double2 s = (i == c) ? make_double2(1.0, 0.0) : czero();
-      for (int k = i + 1; k <= c; ++k) s = csub(s, cmul(A[i * ld + k], X[k * ld + c]));
-      X[i * ld + c] = cscale(s, 1.0 / A[i * ld + i].x);
-    }
+  for (int e = tid; e < nb * nb; e += kQrBlock) {
+    const int i = e / nb, j = e - i * nb;
+    X[i * ld + j] = czero();
   }
   __syncthreads();
+  for (int st = 0; st < nb + 3; ++st) {
+    if (st < nb) {  // (a)
+      const double piv = A[st * ld + st].x;
+      const double inv = 1.0 / piv;
+      if (tid == 0)  // NaN for a non-positive pivot: sqrt of a negative number
+        dg[st] = piv > 0.0 ? sqrt(piv) : sqrt(-1.0 - piv * 0.0);
+      const int m = nb - st - 1;
+      for (int e = tid; e < m * m; e += kQrBlock) {
+        const int r = st + 1 + e / m, c = st + 1 + e % m;
+        if (c >= r)
+          A[r * ld + c] = csub(A[r * ld + c],
+                               cscale(cmulc(A[st * ld + r], A[st * ld + c]), inv));
+      }
+    }
+    if (st >= 1 && st <= nb) {  // (b)
+      const int j = st - 1;
+      const double d = dg[j];
+      const double id = 1.0 / d;
+      for (int c = j + tid; c < nb; c += kQrBlock)
+        A[j * ld + c] = c == j ? make_double2(d, 0.0) : cscale(A[j * ld + c], id);
+    }
+    if (st >= 2 && st <= nb + 1) {  // (c)
+      const int x = st - 2, w = nb - x - 1;
+      const double ir = 1.0 / A[x * ld + x].x;
+      for (int e = tid; e < (x + 1) * w; e += kQrBlock) {
+        const int i = e / w, c = x + 1 + e % w;
+        const double2 xi = i == x ? make_double2(ir, 0.0)
+                                  : cscale(X[i * ld + x], -ir);
+        X[i * ld + c] = cadd(X[i * ld + c], cmul(xi, A[x * ld + c]));
+      }
+    }
+    if (st >= 3) {  // (d)
+      const int y = st - 3;
+      const double ir = 1.0 / A[y * ld + y].x;
+      for (int i = tid; i <= y; i += kQrBlock)
+        X[i * ld + y] = i == y ? make_double2(ir, 0.0)
+                               : cscale(X[i * ld + y], -ir);
+    }
+    __syncthreads();
+  }
   for (int e = tid; e < nb * nb; e += kQrBlock) {
     const int i = e / nb, j = e - i * nb;
     out[e] = (j >= i) ? X[i * ld + j] : czero();
+  }
+}
+
+// chol_inv_kernel for nb <= 32 with the matrices in registers: thread
+// (ty, tx) = (tid / 16, tid % 16) owns rows {ty, ty + 16} x columns
+// {tx, tx + 16} of A and of X = R^{-1}.  The same sweep as chol_inv_kernel
+// (right-looking Cholesky; X from the partial sums of X R = I), with the
+// data that crosses threads passed through LDS buffers that are each written
+// once, ahead of their use: rowbuf[s] = the unscaled pivot row s (written
+// by its owners when step s - 1 has finished it), colbuf[y] = the final
+// column y of X (written at the end of step y + 1).  Step s reads only
+// buffers written before its barrier, so its loads need no ordering with
+// its stores: one barrier per step, nb + 1 steps.
+// 1 / sqrt(a): the hardware estimate refined by one Newton step for
+// f(y) = 1/y^2 - a, y <- y (3 - a y^2) / 2; NaN for a <= 0 (or NaN), so
+// that a non-positive pivot poisons the factor as the IEEE sqrt does.
+__device__ __forceinline__ double rsqrt_newton(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  y = y * (1.5 - 0.5 * a * y * y);
+  return a > 0.0 ? y : __builtin_nan("");
+}
+
+__global__ __launch_bounds__(kQrBlock) void chol_inv32_kernel(
+    const double2* __restrict__ S, double2* __restrict__ rinv, int nb,
+    int* __restrict__ flag_out, const int* __restrict__ flag_in) {
+  __shared__ double2 rowbuf[32][33];
+  __shared__ double2 colbuf[32][33];
+  __shared__ double red[kQrBlock / 64];
+  const int64_t p = blockIdx.x;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  double2* out = rinv + p * nb * nb;
+  if (flag_in && flag_in[p]) {  // uniform over the block
+    for (int e = tid; e < nb * nb; e += kQrBlock) {
+      const int i = e / nb, j = e - i * nb;
+      out[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    return;
+  }
+  const double2* Sp = S + p * nb * nb;
+  double2 A[4], X[4];
+  int rr[2], cc[2];
+  rr[0] = ty; rr[1] = ty + 16;
+  cc[0] = tx; cc[1] = tx + 16;
+  double dist = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = rr[q >> 1], c = cc[q & 1];
+    const bool ok = r < nb && c < nb;
+    A[q] = ok ? Sp[r * nb + c] : czero();
+    X[q] = czero();
+    if (ok && r != c)  // distance of the scaled Gram matrix from I
+      dist += (A[q].x * A[q].x + A[q].y * A[q].y) /
+              (Sp[r * nb + r].x * Sp[c * nb + c].x);
+    if (ok && r == 0) rowbuf[0][c] = A[q];
+  }
+  if (flag_out) {
+    const double d2 = block_sum(dist, red);
+    if (tid == 0)  // NaN or a non-positive diagonal compares false
+      flag_out[p] = (d2 < kQrOnePass * kQrOnePass) ? 1 : 0;
+  }
+  __syncthreads();
+  for (int st = 0; st <= nb; ++st) {
+    // Loads, all unconditional (indices clamped into the buffers) so that
+    // they issue back to back; values outside the step's ranges are
+    // selected away below.  Only buffers written before the last barrier
+    // are read.
+    const int sa = st < nb ? st : nb - 1;
+    const int x = st - 2, xa = x < 0 ? 0 : (x < nb ? x : nb - 1);
+    const int y = st - 1, ya = y < 0 ? 0 : (y < nb ? y : nb - 1);
+    const double pst = rowbuf[sa][sa].x, px = rowbuf[xa][xa].x,
+                 py = rowbuf[ya][ya].x;
+    double2 sr[2], sc[2], rx[2], xi[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      sr[a] = rowbuf[sa][rr[a]];
+      sc[a] = rowbuf[sa][cc[a]];
+      rx[a] = rowbuf[xa][cc[a]];
+      xi[a] = colbuf[xa][rr[a]];
+    }
+    const double ys = rsqrt_newton(pst);
+    const double inv = ys * ys;                 // 1 / pivot
+    const double idx = rsqrt_newton(px);        // 1 / R[x][x]
+    const double iy = rsqrt_newton(py);         // 1 / R[y][y]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int a = q >> 1, b = q & 1;
+      const int r = rr[a], c = cc[b];
+      // Cholesky update with pivot row st
+      if (st < nb && r > st && c >= r && c < nb)
+        A[q] = csub(A[q], cscale(cmulc(sr[a], sc[b]), inv));
+      // row x of R into the partial sums of X
+      if (x >= 0 && r <= x && c > x && c < nb)
+        X[q] = cadd(X[q], cmul(xi[a], cscale(rx[b], idx)));
+      // row st + 1 is final: publish it
+      if (st + 1 < nb && r == st + 1 && c >= r && c < nb)
+        rowbuf[st + 1][c] = A[q];
+      // column y of X is complete: finalise and publish it
+      if (y >= 0 && y < nb && c == y && r <= y) {
+        X[q] = r == y ? make_double2(iy, 0.0) : cscale(X[q], -iy);
+        colbuf[y][r] = X[q];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = rr[q >> 1], c = cc[q & 1];
+    if (r < nb && c < nb) out[r * nb + c] = c >= r ? X[q] : czero();
   }
 }
 
@@ -909,6 +1060,12 @@ int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
     hipLaunchKernelGGL(chol_inv_wide_kernel, dim3((unsigned)P),
                        dim3(kCholWideBlock), shm, s, S, rinv, nb, flag_out,
                        flag_in);
+    JRX_LAUNCH_CHECK();
+    return JRX_OK;
+  }
+  if (nb <= 32) {
+    hipLaunchKernelGGL(chol_inv32_kernel, dim3((unsigned)P), dim3(kQrBlock), 0,
+                       s, S, rinv, nb, flag_out, flag_in);
     JRX_LAUNCH_CHECK();
     return JRX_OK;
   }
@@ -991,7 +1148,14 @@ int launch_qr_bwd_small(const double2* G, const double2* rinv, double2* A,
 }
 
 
-constexpr int kApplyRows = 256;  // rows per block of the apply kernels
+// Rows per block of the narrow apply kernels: 256, halved (down to 64)
+// while the grid would hold fewer than 2048 blocks, so that problems with
+// few k-points still spread over the chip.
+inline int apply_rows(int ng, int64_t P) {
+  int rows = 256;
+  while (rows > 64 && (int64_t)ceil_div(ng, rows) * P < 2048) rows >>= 1;
+  return rows;
+}
 
 // Q[p][b][g] = sum_i W[p][g][i] rinv[p][i][b], computed as Q^T = R^T W^T so
 // that each accumulator row is 16 consecutive g of one band.
@@ -999,7 +1163,7 @@ template <int T>
 __global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
     const double* __restrict__ wr, const double* __restrict__ wi,
     const double2* __restrict__ rinv, double2* __restrict__ q, int ng,
-    int nb) {
+    int nb, int rows) {
   extern __shared__ double2 sm[];
   constexpr int NBP = 16 * T;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1010,8 +1174,8 @@ __global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
   const double* Wr = wr + p * ng * nb;
   const double* Wi = wi + p * ng * nb;
   double2* Q = q + p * nb * ng;
-  const int gb = blockIdx.x * kApplyRows;
-  const int ge = min(ng, gb + kApplyRows);
+  const int gb = blockIdx.x * rows;
+  const int ge = min(ng, gb + rows);
   const int nm = (nb + 3) / 4;
   for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
     d4 qre[T], qim[T];
@@ -1067,12 +1231,13 @@ int launch_apply_fwd(const double* wr, const double* wi, const double2* rinv,
     return JRX_ERR_INVALID;
   }
   size_t shm = (size_t)(16 * T) * (16 * T) * sizeof(double2);
-  dim3 grid(ceil_div(ng, kApplyRows), (unsigned)P), block(kQrBlock);
+  const int rows = apply_rows(ng, P);
+  dim3 grid(ceil_div(ng, rows), (unsigned)P), block(kQrBlock);
   switch (T) {
-    case 1: hipLaunchKernelGGL((apply_fwd_kernel<1>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
-    case 2: hipLaunchKernelGGL((apply_fwd_kernel<2>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
-    case 3: hipLaunchKernelGGL((apply_fwd_kernel<3>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
-    case 4: hipLaunchKernelGGL((apply_fwd_kernel<4>), grid, block, shm, s, wr, wi, rinv, q, ng, nb); break;
+    case 1: hipLaunchKernelGGL((apply_fwd_kernel<1>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, rows); break;
+    case 2: hipLaunchKernelGGL((apply_fwd_kernel<2>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, rows); break;
+    case 3: hipLaunchKernelGGL((apply_fwd_kernel<3>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, rows); break;
+    case 4: hipLaunchKernelGGL((apply_fwd_kernel<4>), grid, block, shm, s, wr, wi, rinv, q, ng, nb, rows); break;
     default:
       set_error("nb above 64 is not supported by this build");
       return JRX_ERR_UNSUPPORTED;
@@ -1087,7 +1252,8 @@ template <int T>
 __global__ __launch_bounds__(kQrBlock) void apply_bwd_kernel(
     const double2* __restrict__ qbar, const double2* __restrict__ q,
     const double2* __restrict__ A, const double2* __restrict__ Bm,
-    double* __restrict__ dwr, double* __restrict__ dwi, int ng, int nb) {
+    double* __restrict__ dwr, double* __restrict__ dwi, int ng, int nb,
+    int rows) {
   extern __shared__ double2 sm[];
   constexpr int NBP = 16 * T;
   double2* sa = sm;
@@ -1102,8 +1268,8 @@ __global__ __launch_bounds__(kQrBlock) void apply_bwd_kernel(
   const double2* Q = q + p * nb * ng;
   double* Dr = dwr + p * ng * nb;
   double* Di = dwi + p * ng * nb;
-  const int gb = blockIdx.x * kApplyRows;
-  const int ge = min(ng, gb + kApplyRows);
+  const int gb = blockIdx.x * rows;
+  const int ge = min(ng, gb + rows);
   const int nm = (nb + 3) / 4;
   for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
     d4 re[T], im[T];
@@ -1174,12 +1340,13 @@ int launch_apply_bwd(const double2* qbar, const double2* q, const double2* A,
     return JRX_OK;
   }
   size_t shm = 2 * (size_t)(16 * T) * (16 * T) * sizeof(double2);
-  dim3 grid(ceil_div(ng, kApplyRows), (unsigned)P), block(kQrBlock);
+  const int rows = apply_rows(ng, P);
+  dim3 grid(ceil_div(ng, rows), (unsigned)P), block(kQrBlock);
   switch (T) {
-    case 1: hipLaunchKernelGGL((apply_bwd_kernel<1>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
-    case 2: hipLaunchKernelGGL((apply_bwd_kernel<2>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
-    case 3: hipLaunchKernelGGL((apply_bwd_kernel<3>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
-    case 4: hipLaunchKernelGGL((apply_bwd_kernel<4>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb); break;
+    case 1: hipLaunchKernelGGL((apply_bwd_kernel<1>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb, rows); break;
+    case 2: hipLaunchKernelGGL((apply_bwd_kernel<2>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb, rows); break;
+    case 3: hipLaunchKernelGGL((apply_bwd_kernel<3>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb, rows); break;
+    case 4: hipLaunchKernelGGL((apply_bwd_kernel<4>), grid, block, shm, s, qbar, q, A, Bm, dwr, dwi, ng, nb, rows); break;
     default:
       set_error("nb above 64 is not supported by this build");
       return JRX_ERR_UNSUPPORTED;
