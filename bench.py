@@ -21,7 +21,7 @@ path runs on one GPU), --steps K, --warmup W, --no-cpu-baseline,
 --traffic-probe (the workload of the rocprofv3 counter runs),
 --no-large-cell, --large-cell-steps K, --large-cell-probe (the 128^3 /
 128-band section and its rocprofv3 workload), --no-small-cell,
---no-band-structure.
+--no-band-structure, --band-probe (the band-mode rocprofv3 workload).
 
 A third section, "batch", times Adam steps (energy + gradient + update) of
 16 uneven structures run as one batch: eager, over several streams, and as
@@ -98,6 +98,54 @@ def cpu_host():
     pass
   return {"nproc": os.cpu_count(), "cpu_model": model,
           "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_band_step(target_s, nthreads):
+  """Band-mode steps (trace and gradient, oracle/band.py) of the Si band
+  example's first k-point on the host, per second; the same synthetic
+  density as band_structure_bench, built on the host by the oracle."""
+  sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+  import numpy as np
+  from threadpoolctl import threadpool_limits
+  from jrystal_amd import band
+  from oracle import adjoint as OA
+  from oracle import band as OB
+  from oracle import grid as OG
+  from oracle import pw as OP
+  from oracle.crystal import Crystal
+  OP.set_workers(nthreads)
+  limits = threadpool_limits(limits=nthreads)
+  cr = Crystal.create_from_file(os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "jrystal_amd", "geometry",
+    "si.xyz"))
+  gs = [64, 64, 64]
+  mask = OG.spherical_mask(cr.cell_vectors, gs, 100.0)
+  g = OG.g_vectors(cr.cell_vectors, gs)
+  k0 = band.get_k_path(cr.cell_vectors, "LGXL", 60)[0:1]
+  gsprob = OA.Problem(mask=mask, g_vector_grid=g, kpts=np.zeros((1, 3)),
+                      vol=cr.vol, occupation=2.0 * np.ones((1, 1, 14)),
+                      positions=cr.positions,
+                      charges=np.asarray(cr.charges, float))
+  rho = OA.energy_and_grad(OP.param_init(0, 14, 1, mask, dist="uniform"),
+                           gsprob, want_grad=False)[2]
+  prob = OA.Problem(mask=mask, g_vector_grid=g, kpts=k0, vol=cr.vol,
+                    occupation=np.ones((1, 1, 28)), positions=cr.positions,
+                    charges=np.asarray(cr.charges, float))
+  v = OB.effective_potential(rho, prob)
+  p = OP.param_init(123, 28, 1, mask, dist="uniform")
+  OB.hamiltonian_trace(p, prob, v)  # warm-up
+  n, t0 = 0, time.perf_counter()
+  while time.perf_counter() - t0 < target_s:
+    OB.hamiltonian_trace(p, prob, v)
+    n += 1
+  t = time.perf_counter() - t0
+  limits.unregister()
+  return {"value": n / t, "unit": "band-mode trace+gradient evaluations/s",
+          "cores": nthreads, "kind": "port",
+          "sample": (f"{n} evaluations ({t:.2f} s) of the Si band example's "
+                     f"first k-point (12939 coefficients x 28 bands, 64^3); "
+                     f"oracle/ NumPy+SciPy FP64, scipy.fft workers="
+                     f"{nthreads}")}
 
 
 def cpu_baseline(wl, params, target_s, nthreads):
@@ -343,30 +391,24 @@ def small_cell_bench(steps, warmup):
   return res
 
 
-def band_structure_bench(adam_first=10000, adam_rest=400, num_kpoints=60):
-  """The reference's all-electron Si band example (tests/all_electron/si/
-  config.yaml: geometry/si.xyz, 64^3, spherical 100 Ha, 28 bands, the
-  60-point path LGXL, Adam with lr 0.001, 10000 steps at the first k-point
-  and 400 at each of the others) through the band driver, from a synthetic
-  fixed density (that of 14 seeded orbitals at Gamma; the cost of a band
-  step does not depend on the density's values): the whole Adam schedule,
-  and the same path with preconditioned CG at each k-point."""
-  import tempfile
-  from dataclasses import dataclass
+BAND_WORKLOAD = "si-fcc-64^3-100Ha-28b-LGXL-60k"
+
+
+def _band_setup(num_kpoints=60, adam_first=10000, adam_rest=400):
+  """(config, driver module, density, mask) of the reference's all-electron
+  Si band example (tests/all_electron/si/config.yaml: geometry/si.xyz,
+  64^3, spherical 100 Ha, 28 bands, the 60-point path LGXL, Adam with
+  lr 0.001, 10000 steps at the first k-point and 400 at each of the
+  others), with a synthetic fixed density: that of 14 seeded orbitals at
+  Gamma (the cost of a band step does not depend on the density's
+  values)."""
   import importlib
+  import tempfile
   import numpy as np
-  import torch
   from jrystal_amd import config as C
   from jrystal_amd import pw
   from jrystal_amd.objective import EnergyPlan
   BD = importlib.import_module("jrystal_amd.calc.band_all_electrons")
-
-  @dataclass
-  class _GS:
-    density: object
-    total_energy: float = 0.0
-
-  tmp = tempfile.mkdtemp(prefix="jrx_band_")
   cfg = C.get_config()
   cfg.update(crystal="si", grid_sizes=64, cutoff_energy=100.0,
              freq_mask_method="spherical", empty_bands=14,
@@ -375,7 +417,8 @@ def band_structure_bench(adam_first=10000, adam_rest=400, num_kpoints=60):
              band_structure_epoch=adam_first,
              k_path_fine_tuning_epoch=adam_rest, optimizer="adam",
              optimizer_args={"learning_rate": 0.001, "b1": 0.9, "b2": 0.99},
-             seed=123, save_dir=tmp, verbose=False)
+             seed=123, save_dir=tempfile.mkdtemp(prefix="jrx_band_"),
+             verbose=False)
   cr = BD.create_crystal(cfg)
   mask = BD.create_freq_mask(cfg, cr)
   gsp = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, mask,
@@ -384,21 +427,51 @@ def band_structure_bench(adam_first=10000, adam_rest=400, num_kpoints=60):
   rho = gsp.evaluate(p0["w_re"], p0["w_im"], grad=False,
                      density=True)["density"].clone()
   gsp.close()
-  print("bench: band structure (Adam schedule)", file=sys.stderr, flush=True)
-  adam = BD.calc_with_output(cfg, ground_state=_GS(rho))
-  res = {"workload": "si-fcc-64^3-100Ha-28b-LGXL-60k",
+  return cfg, BD, rho, mask
+
+
+class _GroundState:
+  def __init__(self, density):
+    self.density = density
+    self.total_energy = 0.0
+
+
+def band_structure_bench(adam_first=10000, adam_rest=400, num_kpoints=60,
+                         chunked=0):
+  """The band driver on the reference's Si band example (_band_setup): the
+  whole Adam schedule on one chunk (the reference on one device), with
+  ``chunked`` > 1 also on that many chunks at once (the reference's
+  parallel_over_k_path on that many devices), and the path with
+  preconditioned CG at each k-point."""
+  import numpy as np
+  import torch
+  cfg, BD, rho, mask = _band_setup(num_kpoints, adam_first, adam_rest)
+  _GS = _GroundState
+  res = {"workload": BAND_WORKLOAD,
          "num_g": int(np.sum(np.asarray(mask))), "num_bands": 28,
          "num_kpoints": num_kpoints, "density": "synthetic (14 seeded "
-         "orbitals at Gamma)",
-         "adam": {"steps": adam_first + (num_kpoints - 1) * adam_rest,
-                  "evaluations": adam.evaluations,
-                  "seconds": adam.seconds,
-                  "ms_per_step": 1e3 * adam.seconds / adam.evaluations}}
+         "orbitals at Gamma)"}
+  runs = {}
+  for chunks in (1, chunked) if chunked > 1 else (1,):
+    print(f"bench: band structure (Adam schedule, {chunks} chunk(s))",
+          file=sys.stderr, flush=True)
+    cfg.update(k_path_chunks=chunks)
+    r = BD.calc_with_output(cfg, ground_state=_GS(rho))
+    per = num_kpoints // chunks
+    res[f"adam_{chunks}_chunk" + ("s" if chunks > 1 else "")] = {
+      "k_path_chunks": chunks,
+      "steps": adam_first + (per - 1) * adam_rest,
+      "evaluations": r.evaluations, "seconds": r.seconds,
+      "ms_per_step": 1e3 * r.seconds / r.evaluations,
+      "k_points_per_s": num_kpoints / r.seconds}
+    runs[chunks] = r
+  adam = runs[1]
   cfg.update(optimizer="cg", band_structure_epoch=400,
-             k_path_fine_tuning_epoch=200)
+             k_path_fine_tuning_epoch=200, k_path_chunks=1)
   print("bench: band structure (CG)", file=sys.stderr, flush=True)
   cg = BD.calc_with_output(cfg, ground_state=_GS(rho))
   res["cg"] = {"evaluations": cg.evaluations, "seconds": cg.seconds,
+               "k_points_per_s": num_kpoints / cg.seconds,
                "max_iter": [400, 200]}
   d = np.abs(adam.eigenvalues - cg.eigenvalues)
   res["max_abs_eig_diff_adam_vs_cg_Ha"] = float(np.max(d))
@@ -488,6 +561,30 @@ def measured_mfma():
     return None
   end = text.find("```", start + 7)
   return json.loads(text[start + 7:end])
+
+
+def band_probe(steps=100):
+  """The workload of the band-mode rocprofv3 runs: ``steps`` Adam steps of
+  the band driver's objective at the first k-point of the Si band example
+  (one band-mode evaluation and the Adam kernel each)."""
+  import torch
+  from jrystal_amd import band, pw
+  from jrystal_amd.hamiltonian import band_plan
+  from jrystal_amd.optim import Adam
+  cfg, BD, rho, mask = _band_setup()
+  cr = BD.create_crystal(cfg)
+  k = band.get_k_path(cr.cell_vectors, "LGXL", 60)[0:1]
+  plan = band_plan(cr.positions, cr.charges, cr.cell_vectors, mask, rho, k,
+                   28)
+  params = pw.param_init(123, 28, 1, mask)
+  opt = Adam(params, **cfg.optimizer_args)
+  out = {}
+  for _ in range(steps):
+    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+    opt.step(params, {"w_re": out["dw_re"], "w_im": out["dw_im"]})
+  torch.cuda.synchronize()
+  print(json.dumps({"probe": BAND_WORKLOAD, "steps": steps,
+                    "trace_Ha": float(out["energies"][4])}))
 
 
 def large_cell_probe():
@@ -612,6 +709,8 @@ def main():
                   help="run the rocprofv3 counter workload and exit")
   ap.add_argument("--no-small-cell", action="store_true")
   ap.add_argument("--no-band-structure", action="store_true")
+  ap.add_argument("--band-probe", action="store_true",
+                  help="only the band-mode rocprofv3 workload")
   ap.add_argument("--no-large-cell", action="store_true")
   ap.add_argument("--large-cell-steps", type=int, default=5)
   ap.add_argument("--large-cell-probe", action="store_true",
@@ -619,6 +718,9 @@ def main():
   args = ap.parse_args()
   if args.traffic_probe:
     traffic_probe()
+    return
+  if args.band_probe:
+    band_probe()
     return
   if args.large_cell_probe:
     large_cell_probe()
@@ -743,6 +845,9 @@ def main():
     c4 = cpu_baseline(wl, params, args.cpu_seconds / 2, 4)
     cpu["value_4_threads"] = c4["value"]
     cpu["sample_4_threads"] = c4["sample"]
+    if bands is not None:
+      cpu["band_structure_step"] = cpu_band_step(args.cpu_seconds / 2,
+                                                 nthreads)
     if conv is not None:
       # CPU seconds per evaluation of the convergence cell, times the
       # evaluations the plain method needed on the GPU

@@ -21,6 +21,13 @@ an Adam step is one fused evaluation plus the fused Adam kernel, with no
 host synchronisation.  optimizer: cg replaces the Adam steps at each
 k-point by preconditioned conjugate gradients on the same objective, at
 most as many iterations as the Adam schedule has steps.
+
+k_path_chunks = D (a key of this implementation, default 1) is the
+reference's parallel_over_k_path on D devices (:115,184-193): the path is
+cut into D contiguous chunks, each starting from the same initial
+parameters and Adam state and walked with its own warm start; here the D
+chunks run as D k-points of one plan, so every kernel sees D problems at
+once.  D = 1 is the reference on one device.
 """
 import logging
 import math
@@ -48,12 +55,13 @@ class BandStructureOutput:
   crystal: Crystal
   k_path: np.ndarray           # (nk, 3) 1/bohr
   eigenvalues: np.ndarray      # (spin, nk, band) Ha
-  params_pw: dict              # the last k-point's parameters
+  params_pw: dict              # each chunk's last parameters (s, chunk, g, b)
   ground_state_energy: float
   density: object = None
   seconds: float = 0.0
-  evaluations: int = 0
+  evaluations: int = 0           # of each chunk's problem
   save_path: str = ""
+  k_path_chunks: int = 1
 
 
 def k_path_of(config, crystal) -> np.ndarray:
@@ -103,14 +111,22 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
   rho = torch.as_tensor(ground_state.density, dtype=torch.float64)
   mask = create_freq_mask(config, cr)
   num_bands = math.ceil(cr.num_electron / 2) + config.band_structure_empty_bands
+  nk = k_path.shape[0]
+  chunks = int(config.get("k_path_chunks") or 1)
+  if chunks < 1 or nk % chunks:
+    raise ValueError(f"{nk} k-points do not split into {chunks} chunks")
+  per = nk // chunks
+  kp = k_path.reshape(chunks, per, 3)  # chunk d holds k[d*per:(d+1)*per]
   plan = band_plan(cr.positions, cr.charges, cr.cell_vectors, mask, rho,
-                   k_path[0:1], num_bands, xc=config.xc,
+                   kp[:, 0], num_bands, xc=config.xc,
                    chunk_states=config.chunk_states)
   params = pw.param_init(config.seed, num_bands, 1, mask,
                          spin_restricted=config.spin_restricted,
                          dist=config.param_dist)
+  # every chunk starts from the same parameters (the reference stacks them
+  # per device, :186-188)
   params = {k: torch.as_tensor(v, dtype=torch.float64).to(plan.device)
-            .contiguous() for k, v in params.items()}
+            .repeat(1, chunks, 1, 1).contiguous() for k, v in params.items()}
   if params["w_re"].shape[0] != plan.ns:
     raise ValueError(f"the ground-state density has {plan.ns} spin "
                      f"channel(s), the band parameters "
@@ -119,14 +135,14 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
   log.info("Number of bands: %d", num_bands)
   opt = Adam(params, **dict(config.optimizer_args))
   out = {}
-  eig = torch.empty((plan.ns, k_path.shape[0], num_bands),
-                    dtype=torch.float64, device=plan.device)
+  eig = torch.empty((plan.ns, chunks, per, num_bands), dtype=torch.float64,
+                    device=plan.device)
   evals = 0
   torch.cuda.synchronize()
   t0 = time.perf_counter()
-  for i in range(k_path.shape[0]):
+  for i in range(per):
     if i > 0:
-      plan.set_kpoints(k_path[i:i + 1])
+      plan.set_kpoints(kp[:, i])
     steps = (config.band_structure_epoch if i == 0
              else config.k_path_fine_tuning_epoch)
     if config.optimizer == "adam":
@@ -135,17 +151,17 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
       evals += _cg_steps(plan, params, steps)
     plan.evaluate(params["w_re"], params["w_im"], grad=False, out=out)
     plan.subspace_eigenvalues(out=out.setdefault(
-      "eig", torch.empty((plan.ns, 1, num_bands), dtype=torch.float64,
+      "eig", torch.empty((plan.ns, chunks, num_bands), dtype=torch.float64,
                          device=plan.device)))
-    eig[:, i] = out["eig"][:, 0]
+    eig[:, :, i] = out["eig"]
     evals += 1
     if config.verbose and (i == 0 or (i + 1) % 10 == 0):
-      log.info("k-point %d/%d | trace %.8f Ha", i + 1, k_path.shape[0],
-               float(out["energies"][4]))
+      log.info("k-point %d/%d of each of %d chunk(s) | trace %.8f Ha", i + 1,
+               per, chunks, float(out["energies"][4]))
   torch.cuda.synchronize()
   seconds = time.perf_counter() - t0
   log.info("Band structure calculation: %.2f seconds", seconds)
-  ev = eig.cpu().numpy()
+  ev = eig.reshape(plan.ns, nk, num_bands).cpu().numpy()
   save_file = "".join(cr.symbols) + "_band_structure.npy"
   if config.save_dir:
     os.makedirs(config.save_dir, exist_ok=True)
@@ -155,7 +171,8 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
   return BandStructureOutput(
     config=config, crystal=cr, k_path=k_path, eigenvalues=ev,
     params_pw=params, ground_state_energy=ground_state.total_energy,
-    density=rho, seconds=seconds, evaluations=evals, save_path=save_file)
+    density=rho, seconds=seconds, evaluations=evals, save_path=save_file,
+    k_path_chunks=chunks)
 
 
 def calc(config: JrystalConfigDict) -> np.ndarray:

@@ -55,6 +55,9 @@ DEFAULT_CONFIG = {
   "chunk_states": 0,         # FFT chunk of the fused evaluation
   "checkpoint_every": 0,     # with save_dir: a checkpoint every N steps
                              # (0: only at the end of the run)
+  "k_path_chunks": 1,        # band structure: the path in D chunks at once
+                             # (the reference's parallel_over_k_path on D
+                             # devices)
 }
 
 

@@ -99,12 +99,20 @@ def test_entropy_matches_oracle():
     assert entropy.fermi_dirac(occ) == OO.fermi_dirac_entropy(occ)
 
 
-def test_cli_band_mode_is_outside_the_path(tmp_path):
+def test_cli_band_mode_paths(tmp_path):
+  """Band mode: norm-conserving is outside this path; the all-electron
+  driver fails loudly without a GPU (no CPU fallback)."""
+  torch = pytest.importorskip("torch")
   from jrystal_amd.__main__ import main
   p = tmp_path / "c.yaml"
-  p.write_text("verbose: false\n")
+  p.write_text("verbose: false\nuse_pseudopotential: true\n")
   with pytest.raises(NotImplementedError):
     main(["-m", "band", "-c", str(p)])
+  if not torch.cuda.is_available():
+    q = tmp_path / "d.yaml"
+    q.write_text("verbose: false\nepoch: 1\n")
+    with pytest.raises(RuntimeError, match="GPU"):
+      main(["-m", "band", "-c", str(q)])
 
 
 def test_driver_rejects_unsupported_options():
