@@ -238,6 +238,34 @@ int jrx_plan_subspace_eigenvalues(const jrx_plan* plan, double* evals,
                                   size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Norm-conserving pseudopotentials.
+ *
+ * jrx_plan_set_external_potential: v_rec [x][y][z] (device, complex) is a
+ *   local potential on the G box in the normalisation of
+ *   external_reciprocal (potential.py:121-166), i.e. the reference's
+ *   potential_local_reciprocal (pseudopotential/local.py:34-112); it takes
+ *   the place of the Coulomb nuclei in the external energy and potential
+ *   (energy_local, local.py:158-187).  v_rec = NULL restores the nuclei.
+ * jrx_plan_set_nonlocal: proj [nk][nproj][ng] (complex; host or device) are
+ *   the non-local projectors at the mask points (the mask restriction of
+ *   potential_nonlocal_psi_reciprocal, nloc.py:58-150).  Later evaluations
+ *   add E_nl = sum_{s,k,b} f sum_p |F_pb|^2 / vol, F = sum_g proj_pg q_gb
+ *   (energy_nonlocal, nloc.py:196-227) to energies[2] (external) and the
+ *   total, and its gradient; in band mode the projectors act in the
+ *   Hamiltonian as well.  nproj = 0 switches them off.  nproj * nb <= 4096
+ *   and nb <= 128.  The first call allocates the projector table and its
+ *   scratch (plan-owned: evaluations of one plan with projectors must not
+ *   overlap on different streams).
+ * jrx_plan_nonlocal_energies: out [s][k][b] = sum_p |F_pb|^2 / vol of the
+ *   last evaluation (per band, without the occupation). */
+int jrx_plan_set_external_potential(jrx_plan* plan, const double* v_rec,
+                                    void* stream);
+int jrx_plan_set_nonlocal(jrx_plan* plan, int nproj, const double* proj,
+                          void* stream);
+int jrx_plan_nonlocal_energies(const jrx_plan* plan, double* out,
+                               void* stream);
+
+/* ------------------------------------------------------------------------
  * Learned preconditioner and warm start (no counterpart in the reference).
  * A FP32 MLP, 8 feature slots -> tanh 64 -> tanh 64 -> 2 outputs, with
  * flat parameters (jrx_mlp_num_params() floats: W1[64][8], b1[64],

@@ -119,6 +119,9 @@ SIGNATURES = {
   "jrx_plan_rotate": [_P, _P, _P, _P, _P, _SZ, _P],
   "jrx_plan_ritz_rotation": [_P, _P, _P, _SZ, _P],
   "jrx_plan_set_fixed_density": [_P, _P, _P, _SZ, _P],
+  "jrx_plan_set_external_potential": [_P, _P, _P],
+  "jrx_plan_set_nonlocal": [_P, _I, _P, _P],
+  "jrx_plan_nonlocal_energies": [_P, _P, _P],
   "jrx_plan_set_kpoints": [_P, _P, _P],
   "jrx_plan_subspace_hamiltonian": [_P, _P, _P, _SZ, _P],
   "jrx_plan_subspace_eigenvalues": [_P, _P, _P, _P, _SZ, _P],

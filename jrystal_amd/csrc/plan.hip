@@ -12,6 +12,7 @@
 #include "energy_kernels.h"
 #include "fft_kernels.h"
 #include "mlp_core.h"
+#include "nonlocal_kernels.h"
 #include "qr_kernels.h"
 
 namespace jrx {
@@ -121,6 +122,16 @@ struct jrx_plan {
   // layout, ns * N) and form sum f <psi|T + v|psi>; null = ground state
   double* d_vfix = nullptr;
   bool fixed = false;
+  // norm-conserving pseudopotential: the non-local projectors
+  // [nk][nproj][ng] and their scratch (plan-owned, so evaluations of one
+  // plan with projectors are not re-entrant across streams); a custom local
+  // potential lives in d_sterm (custom_vloc)
+  int nproj = 0;
+  double2* d_proj = nullptr;
+  void* d_nlscratch = nullptr;
+  double* d_enl = nullptr;  // per-state non-local energies
+  size_t nl_cap = 0, nl_scratch_cap = 0;
+  bool custom_vloc = false;
   void* static_ws = nullptr;  // device address of `scratch` on this device
   int xc = JRX_XC_LDA_X;    // JRX_XC_* terms
   bool rowpath = false;     // row-buffer FFT path (z-major rho and v)
@@ -700,10 +711,12 @@ extern "C" int jrx_plan_band_norms(const jrx_plan* p, double* out,
 
 __global__ void band_energy_kernel(const double* __restrict__ tb,
                                    const double* __restrict__ pot,
+                                   const double* __restrict__ enl,
                                    double two_over_vol, int64_t S,
                                    double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < S) out[i] = 0.5 * (tb[i] + two_over_vol * pot[i]);
+  if (i < S)
+    out[i] = 0.5 * (tb[i] + two_over_vol * pot[i]) + (enl ? enl[i] : 0.0);
 }
 
 extern "C" int jrx_plan_set_band_energies(jrx_plan* p, int on) {
@@ -723,7 +736,8 @@ extern "C" int jrx_plan_band_energies(const jrx_plan* p, double* out,
                      0, (hipStream_t)stream,
                      reinterpret_cast<const double*>(ws + p->o_tb),
                      reinterpret_cast<const double*>(ws + p->o_pot),
-                     2.0 / p->vol, p->S, out);
+                     p->nproj > 0 ? p->d_enl : nullptr, 2.0 / p->vol, p->S,
+                     out);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -1314,6 +1328,11 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
     if (pot_e)
       JRX_T(JRX_K_OTHER, launch_state_sums(dptr(p->o_bepart), nper, S,
                                            dptr(p->o_pot), s));
+    // --- non-local pseudopotential: energy per band and its qbar term ------
+    if (p->nproj > 0)
+      JRX_T(JRX_K_OTHER, launch_nonlocal(p->d_proj, q, p->d_occ, P, p->nk,
+                                         p->nproj, ng, nb, p->vol, qbar,
+                                         p->d_nlscratch, p->d_enl, s));
     // --- QR adjoint (Q^H Qbar alone without a gradient: the subspace
     // matrix of jrx_plan_subspace_eigenvalues) --------------------------------
     JRX_T(JRX_K_QR_BWD_GRAM, launch_gram_bmajor(q, qbar, part, P, ng, nb, p->gsq, s));
@@ -1326,6 +1345,10 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
     JRX_T(JRX_K_OTHER, launch_kinetic(q, p->d_occ, p->d_kpts, p->d_g_m, p->cp, S, ng,
                            p->nk, nb, kin, &nkin, s));
     JRX_T(JRX_K_OTHER, launch_state_sums(kin, (int)(nkin / S), S, tb, s));
+    if (p->nproj > 0)
+      JRX_T(JRX_K_OTHER, launch_nonlocal(p->d_proj, q, p->d_occ, P, p->nk,
+                                         p->nproj, ng, nb, p->vol, nullptr,
+                                         p->d_nlscratch, p->d_enl, s));
   }
   if (fixed) {
     hipLaunchKernelGGL(band_trace_kernel, dim3(1), dim3(256), 0, s, tb,
@@ -1335,6 +1358,9 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
     JRX_T(JRX_K_OTHER, launch_finalize_energies(tb, p->d_occ, S, eh, ee, nrec,
                                                 exc, nexc, energies, s));
   }
+  if (p->nproj > 0)  // the non-local energy joins the external term
+    JRX_T(JRX_K_OTHER, launch_nonlocal_finalize(p->d_enl, p->d_occ, S,
+                                                energies, s));
   if (rho_out && p->rowpath) {
     if ((rc = launch_zmajor_to_xmajor(rho, rho_out, p->ns, p->nx, s)))
       return rc;
@@ -1442,6 +1468,99 @@ int jrx_plan_subspace_hamiltonian(const jrx_plan* p, double* h,
   return launch_subspace_h(reinterpret_cast<const double2*>(ws + p->o_S),
                            p->d_occ, p->P, p->nb, reinterpret_cast<double2*>(h),
                            (hipStream_t)stream);
+}
+
+// sterm[i] = conj(v[x-major index of i]) * scale on the plan's box (on the
+// row path the box axes are (z, y, x) of a cubic n^3 box).
+__global__ void vloc_to_sterm_kernel(const double2* __restrict__ v,
+                                     double2* __restrict__ sterm, int64_t N,
+                                     int n, int zmajor, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int64_t src = i;
+  if (zmajor) {
+    const int64_t a = i / ((int64_t)n * n), rem = i - a * n * n;
+    const int64_t b = rem / n, c = rem - b * n;  // i = (a, b, c) = (z, y, x)
+    src = (c * n + b) * n + a;
+  }
+  const double2 t = v[src];
+  sterm[i] = make_double2(t.x * scale, -t.y * scale);
+}
+
+static int realloc_dev(jrx_plan* p, void** ptr, size_t* cap, size_t bytes) {
+  if (bytes <= *cap && *ptr) return JRX_OK;
+  if (*ptr) {
+    for (size_t i = 0; i < p->allocs.size(); ++i)
+      if (p->allocs[i] == *ptr) {
+        p->allocs.erase(p->allocs.begin() + i);
+        break;
+      }
+    JRX_CHECK_HIP(hipFree(*ptr));
+    *ptr = nullptr;
+  }
+  JRX_CHECK_HIP(hipMalloc(ptr, bytes));
+  p->allocs.push_back(*ptr);
+  *cap = bytes;
+  return JRX_OK;
+}
+
+int jrx_plan_set_external_potential(jrx_plan* p, const double* v_rec,
+                                    void* stream) {
+  JRX_REQUIRE(p, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  if (!v_rec) {  // back to the Coulomb nuclei
+    int rc = launch_structure_term(p->rowpath ? p->cpz : p->cp, p->d_sterm, s);
+    if (rc) return rc;
+    p->custom_vloc = false;
+    return JRX_OK;
+  }
+  const double scale = -p->vol / (4.0 * M_PI * (double)p->N);
+  hipLaunchKernelGGL(vloc_to_sterm_kernel, dim3(ceil_div(p->N, 256)),
+                     dim3(256), 0, s, reinterpret_cast<const double2*>(v_rec),
+                     p->d_sterm, p->N, p->nx, p->rowpath ? 1 : 0, scale);
+  JRX_LAUNCH_CHECK();
+  p->custom_vloc = true;
+  return JRX_OK;
+}
+
+int jrx_plan_set_nonlocal(jrx_plan* p, int nproj, const double* proj,
+                          void* stream) {
+  JRX_REQUIRE(p && nproj >= 0, "invalid argument");
+  if (nproj == 0) {
+    p->nproj = 0;
+    return JRX_OK;
+  }
+  JRX_REQUIRE(proj, "null projectors");
+  JRX_REQUIRE(p->nb <= 128, "non-local projectors: more than 128 bands");
+  JRX_REQUIRE((int64_t)nproj * p->nb <= 4096,
+              "non-local projectors: projectors x bands above 4096");
+  int rc;
+  const size_t pbytes = (size_t)p->nk * nproj * p->ng * sizeof(double2);
+  if ((rc = realloc_dev(p, reinterpret_cast<void**>(&p->d_proj), &p->nl_cap,
+                        pbytes)))
+    return rc;
+  size_t sc = nl_scratch_bytes(p->P, nproj, p->nb, p->ng);
+  if ((rc = realloc_dev(p, &p->d_nlscratch, &p->nl_scratch_cap, sc)))
+    return rc;
+  if (!p->d_enl) {
+    size_t cap = 0;
+    void* e = nullptr;
+    if ((rc = realloc_dev(p, &e, &cap, (size_t)p->S * sizeof(double))))
+      return rc;
+    p->d_enl = static_cast<double*>(e);
+  }
+  JRX_CHECK_HIP(hipMemcpyAsync(p->d_proj, proj, pbytes, hipMemcpyDefault,
+                               (hipStream_t)stream));
+  p->nproj = nproj;
+  return JRX_OK;
+}
+
+int jrx_plan_nonlocal_energies(const jrx_plan* p, double* out, void* stream) {
+  JRX_REQUIRE(p && out, "null argument");
+  JRX_REQUIRE(p->nproj > 0, "no non-local projectors (jrx_plan_set_nonlocal)");
+  JRX_CHECK_HIP(hipMemcpyAsync(out, p->d_enl, (size_t)p->S * sizeof(double),
+                               hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return JRX_OK;
 }
 
 }  // extern "C"

@@ -354,6 +354,55 @@ class EnergyPlan:
         _lib.stream_ptr(self.device)))
     self.band_mode = density is not None
 
+  def set_external_potential(self, v_rec):
+    """Replace the Coulomb nuclei by a local potential given on the G box
+    (nx,ny,nz) complex, in the normalisation of energy.external_reciprocal
+    (the reference's potential_local_reciprocal, local.py:34-112);
+    None restores the nuclei."""
+    ptr = None
+    if v_rec is not None:
+      v = torch.as_tensor(v_rec).to(device=self.device,
+                                    dtype=torch.complex128).contiguous()
+      if tuple(v.shape) != tuple(self.grid_sizes):
+        raise ValueError(f"external potential has shape {tuple(v.shape)}, "
+                         f"expected {tuple(self.grid_sizes)}")
+      self._vloc = v
+      ptr = _lib.ptr(v)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_set_external_potential(
+        self._plan, ptr, _lib.stream_ptr(self.device)))
+
+  def set_nonlocal(self, projectors):
+    """Non-local pseudopotential projectors (nk, nproj, ng) complex at the
+    mask points (potential_nonlocal_psi_reciprocal restricted to the mask);
+    later evaluations add sum f sum_p |F_pb|^2 / vol, F = P c_b, to the
+    external energy, and its gradient.  None or nproj = 0 switches it off."""
+    if projectors is None:
+      n, ptr = 0, None
+    else:
+      pr = torch.as_tensor(projectors).to(device=self.device,
+                                          dtype=torch.complex128).contiguous()
+      if pr.ndim != 3 or pr.shape[0] != self.nk or pr.shape[2] != self.num_g:
+        raise ValueError(f"projectors must have shape (nk={self.nk}, nproj, "
+                         f"ng={self.num_g}), got {tuple(pr.shape)}")
+      self._proj = pr
+      n, ptr = int(pr.shape[1]), _lib.ptr(pr)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_set_nonlocal(
+        self._plan, n, ptr, _lib.stream_ptr(self.device)))
+    self.num_projectors = n
+
+  def nonlocal_energies(self, out=None):
+    """Per-band non-local energies sum_p |F_pb|^2 / vol (ns,nk,nb) of the
+    last evaluation (without the occupation)."""
+    if out is None:
+      out = torch.empty((self.ns, self.nk, self.nb), dtype=torch.float64,
+                        device=self.device)
+    with torch.cuda.device(self.device):
+      _lib.check(self._lib.jrx_plan_nonlocal_energies(
+        self._plan, _lib.ptr(out), _lib.stream_ptr(self.device)))
+    return out
+
   def set_kpoints(self, kpts):
     """Replace the k-points (nk,3) [1/bohr] of later evaluations."""
     k = torch.as_tensor(np.asarray(kpts, np.float64).reshape(-1, 3))

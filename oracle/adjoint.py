@@ -40,6 +40,11 @@ class Problem:
   positions: np.ndarray     # (atom,3)
   charges: np.ndarray       # (atom,)
   xc: str = "lda_x"
+  # norm-conserving pseudopotential (oracle/pseudopotential.py): the local
+  # potential box in the reference's external_reciprocal normalisation,
+  # replacing the Coulomb nuclei, and the non-local projectors (k, p, g)
+  v_ext_rec: Optional[np.ndarray] = None
+  projectors: Optional[np.ndarray] = None
 
   @property
   def grid_sizes(self):
@@ -58,6 +63,8 @@ def hartree_external_potential(rho_rec, prob: Problem):
   g = prob.g_vector_grid
   vol = prob.vol
   n = np.prod(rho_rec.shape[-3:])
+  if prob.v_ext_rec is not None:  # local pseudopotential
+    v_e = (vol / n) * np.real(sfft.ifftn(prob.v_ext_rec, workers=_pw._WORKERS))
   rho_tot_rec = np.sum(rho_rec, axis=0)
   g2 = np.sum(g**2, axis=-1)
   g2h = g2.copy()
@@ -65,10 +72,11 @@ def hartree_external_potential(rho_rec, prob: Problem):
   vh_rec = 4 * np.pi * rho_tot_rec / g2h
   vh_rec[0, 0, 0] = 0.0
   v_h = (vol / n) * np.real(sfft.ifftn(vh_rec, workers=_pw._WORKERS))
-  s_conj = np.exp(-1j * (g @ prob.positions.T)) @ prob.charges.astype(float)
-  ve_rec = s_conj / (g2 + 1e-10)
-  ve_rec[0, 0, 0] = 0.0
-  v_e = -4 * np.pi * np.real(sfft.ifftn(ve_rec, workers=_pw._WORKERS))
+  if prob.v_ext_rec is None:
+    s_conj = np.exp(-1j * (g @ prob.positions.T)) @ prob.charges.astype(float)
+    ve_rec = s_conj / (g2 + 1e-10)
+    ve_rec[0, 0, 0] = 0.0
+    v_e = -4 * np.pi * np.real(sfft.ifftn(ve_rec, workers=_pw._WORKERS))
   return v_h + v_e
 
 
@@ -81,7 +89,10 @@ def potential_real(rho, prob: Problem):
   n = np.prod(rho.shape[-3:])
   rho_rec = sfft.fftn(rho, axes=(-3, -2, -1), workers=_pw._WORKERS)
   e_har = _en.hartree(rho_rec, g, vol)
-  e_ext = _en.external(rho_rec, prob.positions, prob.charges, g, vol)
+  if prob.v_ext_rec is None:
+    e_ext = _en.external(rho_rec, prob.positions, prob.charges, g, vol)
+  else:  # energy_local (local.py:158-187)
+    e_ext = float(_en.reciprocal_braket(prob.v_ext_rec, rho_rec, vol))
   e_xc = _en.xc_energy(rho, g, vol, prob.xc)
   v_he = hartree_external_potential(rho_rec, prob)
   if prob.xc != "lda_x":
@@ -148,6 +159,13 @@ def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
   absq = np.real(np.conj(q) * q)  # (s,k,g,b)
   parts["kinetic"] = float(np.sum(
     np.einsum("kg,skgb->skb", gk2, absq) * occ) / 2)
+  nl = None
+  if prob.projectors is not None:  # energy_nonlocal (nloc.py:196-227)
+    from . import pseudopotential as _ps
+    nl = _ps.energy_nonlocal(q, prob.projectors, occ, vol, want_grad)
+    parts["external_local"] = parts["external"]
+    parts["external_nonlocal"] = nl[0]
+    parts["external"] = parts["external"] + nl[0]
   parts["total"] = (parts["kinetic"] + parts["hartree"] + parts["external"] +
                     parts["xc"])
   if not want_grad:
@@ -167,6 +185,8 @@ def energy_and_grad(params, prob: Problem, k_chunk: Optional[int] = None,
       hv = _pw.squeeze_coefficient(hv, mask)  # (kc,g,b)
       f = occ[s, kb][:, None, :]
       qbar[s, kb] = f * (2.0 / np.sqrt(vol) * hv + gk2[kb, :, None] * q[s, kb])
+      if nl is not None:
+        qbar[s, kb] += nl[1][s, kb]
   wbar = np.full_like(q, np.nan)
   kall = np.concatenate(blocks)
   wbar[:, kall] = qr_backward(q[:, kall], r[:, kall], qbar[:, kall])
@@ -207,6 +227,9 @@ def band_energies(params, prob: Problem, k_chunk: Optional[int] = None):
   gm = prob.mask_g()
   gk2 = np.sum((gm[None] + prob.kpts[:, None, :])**2, axis=-1)
   out += 0.5 * np.einsum("kg,skgb->skb", gk2, np.real(np.conj(q) * q))
+  if prob.projectors is not None:
+    from . import pseudopotential as _ps
+    out += _ps.energy_nonlocal(q, prob.projectors, occ, prob.vol, False)[2]
   return out
 
 
