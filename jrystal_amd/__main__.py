@@ -21,7 +21,7 @@ def main(argv=None):
                       format="%(message)s")
   if args.mode == "energy":
     if cfg.use_pseudopotential:
-      calc.energy_normcons(cfg)
+      calc.energy_normcons(cfg, resume=args.load)
     else:
       calc.energy_all_electrons(cfg, resume=args.load)
   else:

@@ -16,6 +16,11 @@ Mirrors jrystal/calc/calc_band_structure_all_electrons.py:37-209:
 6. eigenvalues (spin, k, band) saved as <symbols>_band_structure.npy
    (:199-209), in save_dir when it is set.
 
+With use_pseudopotential (norm-conserving) this is
+calc_band_structure_normcons.py:72-299: valence electrons, the local
+pseudopotential inside the fixed potential and the non-local projectors of
+each k-point in the Hamiltonian.
+
 One plan in band mode serves the whole path (set_kpoints moves it along);
 an Adam step is one fused evaluation plus the fused Adam kernel, with no
 host synchronisation.  optimizer: cg replaces the Adam steps at each
@@ -96,9 +101,10 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
                      ) -> BandStructureOutput:
   """ground_state: an energy-driver output to take the density from (None:
   run the energy driver with this config, as the reference does)."""
-  if config.use_pseudopotential:
-    raise NotImplementedError("norm-conserving band structures are outside "
-                              "the all-electron path")
+  if config.use_pseudopotential and config.pseudopotential_type not in (
+      "normcons", "normconserving", "nc"):
+    raise NotImplementedError("band structures: all-electron or "
+                              "norm-conserving (nc) pseudopotentials")
   if config.optimizer not in ("adam", "cg"):
     raise NotImplementedError(f"band-structure optimizer {config.optimizer!r}"
                               ": adam (the reference's) or cg")
@@ -110,16 +116,37 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
     ground_state = ground_state_calc(config)
   rho = torch.as_tensor(ground_state.density, dtype=torch.float64)
   mask = create_freq_mask(config, cr)
-  num_bands = math.ceil(cr.num_electron / 2) + config.band_structure_empty_bands
+  ne = cr.num_electron
+  vloc = proj_path = None
+  if config.use_pseudopotential:
+    # calc_band_structure_normcons.py:85-145: valence electrons, the local
+    # part in the fixed potential, projectors of every path point at once
+    from .. import grid, pseudopotential as PS
+    from .energy_all_electrons import create_pseudopotential
+    pseudo = create_pseudopotential(config, cr)
+    ne = pseudo.total_valence_charge
+    g_vec = grid.g_vectors(cr.cell_vectors,
+                           grid.proper_grid_size(config.grid_sizes))
+    vloc = PS.potential_local_reciprocal(pseudo, g_vec, cr.vol)
+    proj_path = torch.as_tensor(
+      PS.potential_nonlocal_psi_reciprocal(pseudo, g_vec, mask, k_path))
+  num_bands = math.ceil(ne / 2) + config.band_structure_empty_bands
   nk = k_path.shape[0]
   chunks = int(config.get("k_path_chunks") or 1)
   if chunks < 1 or nk % chunks:
     raise ValueError(f"{nk} k-points do not split into {chunks} chunks")
   per = nk // chunks
   kp = k_path.reshape(chunks, per, 3)  # chunk d holds k[d*per:(d+1)*per]
+  def path_projectors(i):  # the projectors of position i of every chunk
+    if proj_path is None:
+      return None
+    return proj_path[[d * per + i for d in range(chunks)]]
   plan = band_plan(cr.positions, cr.charges, cr.cell_vectors, mask, rho,
-                   kp[:, 0], num_bands, xc=config.xc,
+                   kp[:, 0], num_bands, xc=config.xc, local_potential=vloc,
+                   projectors=path_projectors(0),
                    chunk_states=config.chunk_states)
+  if proj_path is not None:
+    proj_path = proj_path.to(plan.device)
   params = pw.param_init(config.seed, num_bands, 1, mask,
                          spin_restricted=config.spin_restricted,
                          dist=config.param_dist)
@@ -143,6 +170,8 @@ def calc_with_output(config: JrystalConfigDict, ground_state=None,
   for i in range(per):
     if i > 0:
       plan.set_kpoints(kp[:, i])
+      if proj_path is not None:
+        plan.set_nonlocal(path_projectors(i))
     steps = (config.band_structure_epoch if i == 0
              else config.k_path_fine_tuning_epoch)
     if config.optimizer == "adam":

@@ -7,6 +7,14 @@ the free energy F = E - T S with the temperature annealing schedule and the
 convergence window.  Each step's energy and gradient come from one fused
 evaluation (jrystal_amd.objective.EnergyPlan); the update is the fused Adam
 kernel.  Returns the density, as the reference does.
+
+With use_pseudopotential (norm-conserving files) the same loop is the
+driver of calc_ground_state_energy_normcons.py:63-250: the valence electrons
+set the bands and occupations, the local pseudopotential replaces the
+nuclei in the fused evaluation and the non-local projectors join it
+(jrystal_amd.pseudopotential); the Ewald term keeps the crystal's nuclear
+charges, as the reference's get_ewald_coulomb_repulsion does
+(opt_utils.py:187-201).
 """
 import logging
 import math
@@ -58,6 +66,24 @@ def create_crystal(config) -> Crystal:
   return cr
 
 
+def create_pseudopotential(config, crystal):
+  """opt_utils.py:117-140 for norm-conserving files: the config's
+  pseudopotential_file_dir, else the shipped directory."""
+  from ..pseudopotential import NormConservingPseudopotential
+  d = config.pseudopotential_file_dir
+  return NormConservingPseudopotential.create(crystal, d if d else None)
+
+
+def apply_pseudopotential(plan, pseudo, g_vec, mask, kpts, vol):
+  """The local part in place of the nuclei and the non-local projectors
+  (calc_ground_state_energy_normcons.py:111-145)."""
+  from .. import pseudopotential as PS
+  plan.set_external_potential(PS.potential_local_reciprocal(pseudo, g_vec,
+                                                            vol))
+  plan.set_nonlocal(PS.potential_nonlocal_psi_reciprocal(pseudo, g_vec, mask,
+                                                         kpts))
+
+
 def create_freq_mask(config, crystal):
   """opt_utils.py:76-103."""
   gs = grid.proper_grid_size(config.grid_sizes)
@@ -97,9 +123,11 @@ def calc_with_output(config: JrystalConfigDict,
   interrupted and resumed Adam run ends bitwise equal to an uninterrupted
   one.  The CG optimisers restart their search direction from the saved
   parameters."""
-  if config.use_pseudopotential:
-    raise NotImplementedError("pseudopotential energies are outside the "
-                              "all-electron path")
+  if config.use_pseudopotential and config.pseudopotential_type not in (
+      "normcons", "normconserving", "nc"):
+    raise NotImplementedError(
+      f"pseudopotential type {config.pseudopotential_type!r}: norm-conserving "
+      "(nc) only")
   if config.occupation not in ("uniform", "gamma") + occupation.PARAMETERISED:
     raise ValueError(f"Invalid method: {config.occupation}")
   parameterised = config.occupation in occupation.PARAMETERISED
@@ -116,6 +144,11 @@ def calc_with_output(config: JrystalConfigDict,
     raise NotImplementedError("Scheduler is not implemented yet.")
   cr = create_crystal(config)
   ne = cr.num_electron
+  pseudo = None
+  if config.use_pseudopotential:
+    # calc_ground_state_energy_normcons.py:82-105: the valence electrons
+    pseudo = create_pseudopotential(config, cr)
+    ne = pseudo.total_valence_charge
   gs = grid.proper_grid_size(config.grid_sizes)
   ks = grid.proper_grid_size(config.k_grid_sizes)
   g_vec = grid.g_vectors(cr.cell_vectors, gs)
@@ -139,6 +172,8 @@ def calc_with_output(config: JrystalConfigDict,
   etro = entropy.fermi_dirac(occ, eps=config.eps)
   plan = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, mask, kpts,
                     occ, xc=config.xc, chunk_states=config.chunk_states)
+  if pseudo is not None:
+    apply_pseudopotential(plan, pseudo, g_vec, mask, kpts, cr.vol)
   params = pw.param_init(config.seed, num_bands, nk, mask,
                          spin_restricted=config.spin_restricted,
                          dist=config.param_dist)
@@ -252,11 +287,20 @@ def _final(config, cr, plan, params, occ, ew, history, converged, steps,
   final = plan.evaluate(params["w_re"], params["w_im"], grad=False,
                         density=True)
   e = dict(zip(ENERGY_KEYS, final["energies"].tolist()))
+  if getattr(plan, "num_projectors", 0):
+    occ_t = torch.as_tensor(np.asarray(plan.occupation, np.float64),
+                            device=plan.device)
+    e["external_nonlocal"] = float(torch.sum(plan.nonlocal_energies() *
+                                             occ_t))
+    e["external_local"] = e["external"] - e["external_nonlocal"]
   e["nuclear_repulsion"] = ew
   e["total_with_nuclear"] = e["total"] + ew
-  for name, key in (("Hartree", "hartree"), ("External", "external"),
-                    ("XC", "xc"), ("Kinetic", "kinetic"),
-                    ("Nuclear repulsion", "nuclear_repulsion")):
+  terms = (("Hartree", "hartree"), ("External", "external"))
+  if "external_nonlocal" in e:
+    terms = (("Hartree", "hartree"), ("External (local)", "external_local"),
+             ("External (nonlocal)", "external_nonlocal"))
+  for name, key in terms + (("XC", "xc"), ("Kinetic", "kinetic"),
+                            ("Nuclear repulsion", "nuclear_repulsion")):
     log.info("%s Energy: %.4f Ha", name, e[key])
   log.info("Total Energy: %.4f Ha", e["total_with_nuclear"])
   return GroundStateEnergyOutput(

@@ -23,9 +23,14 @@ from .objective import EnergyPlan
 
 
 def band_plan(positions, charges, cell_vectors, mask, density, kpts,
-              num_bands: int, xc: str = "lda_x", **options) -> EnergyPlan:
+              num_bands: int, xc: str = "lda_x", local_potential=None,
+              projectors=None, **options) -> EnergyPlan:
   """A plan in band mode: the ground-state ``density`` (ns,nx,ny,nz) fixes
-  the potential, every band has weight one, ``kpts`` (nk,3) 1/bohr."""
+  the potential, every band has weight one, ``kpts`` (nk,3) 1/bohr.  With a
+  norm-conserving pseudopotential, ``local_potential`` (the G box of
+  pseudopotential.potential_local_reciprocal) replaces the nuclei in that
+  potential and ``projectors`` (nk, nproj, ng) add the non-local term, as
+  in calc_band_structure_normcons.py:107-160."""
   rho = torch.as_tensor(density, dtype=torch.float64)
   if rho.ndim == 3:
     rho = rho[None]
@@ -33,7 +38,11 @@ def band_plan(positions, charges, cell_vectors, mask, density, kpts,
   occ = np.ones((rho.shape[0], kpts.shape[0], int(num_bands)))
   plan = EnergyPlan(positions, charges, cell_vectors, mask, kpts, occ, xc=xc,
                     **options)
+  if local_potential is not None:
+    plan.set_external_potential(local_potential)
   plan.set_fixed_density(rho)
+  if projectors is not None:
+    plan.set_nonlocal(projectors)
   return plan
 
 

@@ -23,6 +23,10 @@ For a fixed ground-state density rho0, restates:
   H_GG' = 1/2 |G+k|^2 delta_GG' + v~(G - G'), v~ = fftn(v_eff) / N indexed
   by integer frequency differences modulo the grid: built without the FFT
   application above, for brute-force checks.
+* With norm-conserving pseudopotentials (prob.v_ext_rec, prob.projectors)
+  the local part enters v_eff and the non-local term
+  (1/vol) sum_p conj(P_p(G)) P_p(G') the Hamiltonian, as in the
+  reference's normcons.hamiltonian_trace (pseudopotential/nloc.py:230-279).
 
 xc terms other than LDA exchange use the functional derivative of the
 discrete E_xc (oracle.xc.xc_potential).  For LDA correlation that is the
@@ -91,10 +95,16 @@ def hamiltonian_trace(params, prob: _adj.Problem, v, want_grad: bool = True,
   absq = np.real(np.conj(q) * q)
   kin = 0.5 * np.einsum("kg,skgb->skb", gk2, absq)
   parts = {"kinetic": float(np.sum(occ * kin)),
-           "potential": float(np.sum(occ * pot))}
-  parts["total"] = parts["kinetic"] + parts["potential"]
+           "potential": float(np.sum(occ * pot)), "nonlocal": 0.0}
   grad = None
   hq = 0.5 * gk2[None, :, :, None] * q + hv / np.sqrt(prob.vol)
+  if prob.projectors is not None:  # nloc.py:230-279 (hamiltonian_trace)
+    P = prob.projectors
+    F = np.einsum("kpg,skgb->skpb", P, q)
+    parts["nonlocal"] = float(np.sum(occ * np.sum(np.abs(F)**2, axis=2)) /
+                              prob.vol)
+    hq = hq + np.einsum("skpb,kpg->skgb", F, np.conj(P)) / prob.vol
+  parts["total"] = parts["kinetic"] + parts["potential"] + parts["nonlocal"]
   qbar = 2.0 * occ[:, :, None, :] * hq
   if want_grad:
     wbar = _adj.qr_backward(q, r, qbar)
@@ -129,4 +139,7 @@ def dense_hamiltonian(prob: _adj.Problem, v, s: int = 0, k: int = 0,
   kk = prob.kpts[k] if kpt is None else np.asarray(kpt, float)
   gk2 = np.sum((prob.mask_g() + kk)**2, axis=-1)
   h[np.diag_indices_from(h)] += 0.5 * gk2
+  if prob.projectors is not None and kpt is None:
+    P = prob.projectors[k]
+    h += np.conj(P).T @ P / prob.vol
   return h

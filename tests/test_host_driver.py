@@ -100,12 +100,13 @@ def test_entropy_matches_oracle():
 
 
 def test_cli_band_mode_paths(tmp_path):
-  """Band mode: norm-conserving is outside this path; the all-electron
+  """Band mode: ultrasoft pseudopotentials are outside this path; the
   driver fails loudly without a GPU (no CPU fallback)."""
   torch = pytest.importorskip("torch")
   from jrystal_amd.__main__ import main
   p = tmp_path / "c.yaml"
-  p.write_text("verbose: false\nuse_pseudopotential: true\n")
+  p.write_text("verbose: false\nuse_pseudopotential: true\n"
+               "pseudopotential_type: us\n")
   with pytest.raises(NotImplementedError):
     main(["-m", "band", "-c", str(p)])
   if not torch.cuda.is_available():

@@ -177,3 +177,19 @@ def test_product_tables_match_oracle():
     ga = np.conj(a[k].T) @ a[k]
     gb = np.conj(b[k].T) @ b[k]
     assert rel_max(ga, gb) < 1e-12
+
+
+def test_band_mode_nonlocal_dense_matches_application():
+  """With the pseudopotential's local part in v_eff and the projectors in
+  the Hamiltonian, Q^H H_dense Q equals the FFT/projector application."""
+  from oracle import band as OB
+  cr, pp, prob, params = _si_problem(gs=(10, 10, 10), ecut=8.0, nb=4)
+  prob.occupation = np.ones_like(prob.occupation)
+  q = np.linalg.qr(params["w_re"] + 1j * params["w_im"])[0]
+  rho = OA.energy_and_grad(params, prob, want_grad=False)[2]
+  v = OB.effective_potential(rho, prob)
+  hs = OB.subspace_hamiltonian(params, prob, v)
+  for k in range(prob.kpts.shape[0]):
+    hd = OB.dense_hamiltonian(prob, v, 0, k)
+    assert rel_max(hd, np.conj(hd.T)) < 1e-13
+    assert rel_max(hs[0, k], np.conj(q[0, k].T) @ hd @ q[0, k]) < 1e-12
