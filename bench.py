@@ -21,7 +21,8 @@ path runs on one GPU), --steps K, --warmup W, --no-cpu-baseline,
 --traffic-probe (the workload of the rocprofv3 counter runs),
 --no-large-cell, --large-cell-steps K, --large-cell-probe (the 128^3 /
 128-band section and its rocprofv3 workload), --no-small-cell,
---no-band-structure, --band-probe (the band-mode rocprofv3 workload).
+--no-band-structure, --band-probe (the band-mode rocprofv3 workload),
+--no-normcons, --normcons-probe (the norm-conserving rocprofv3 workload).
 
 A third section, "batch", times Adam steps (energy + gradient + update) of
 16 uneven structures run as one batch: eager, over several streams, and as
@@ -38,6 +39,10 @@ passes.
 A sixth, "band_structure", runs the band driver on the reference's Si band
 example (60-point path, 28 bands, 64^3): its whole Adam schedule, and the
 path with CG at each k-point.
+
+A seventh, "normcons", times evaluations of the reference's
+norm-conserving diamond example (48^3, 8 k, 14 bands, local part and
+non-local projectors).
 """
 import argparse
 import json
@@ -563,6 +568,141 @@ def measured_mfma():
   return json.loads(text[start + 7:end])
 
 
+NC_WORKLOAD = "nc-diamond-48^3-100Ha-8k-14b"
+
+
+def _normcons_setup():
+  """The reference's norm-conserving diamond example
+  (tests/normconserving/diamond/config.yaml: C2 FCC, 48^3, spherical
+  100 Ha, 2x2x2 k, ceil(8/2) + 10 = 14 bands, LDA exchange, uniform
+  occupation of the 8 valence electrons), as a plan with the local part and
+  the projectors set, and seeded parameters."""
+  import numpy as np
+  from jrystal_amd import pseudopotential as PS
+  from jrystal_amd.objective import EnergyPlan
+  params, (cr, pp, g, mask, kpts, occ) = _normcons_inputs()
+  plan = EnergyPlan(cr.positions, cr.charges, cr.cell_vectors, mask, kpts,
+                    occ)
+  plan.set_external_potential(PS.potential_local_reciprocal(pp, g, cr.vol))
+  plan.set_nonlocal(PS.potential_nonlocal_psi_reciprocal(pp, g, mask, kpts))
+  return plan, params, (cr, pp, g, mask, kpts, occ)
+
+
+def _normcons_inputs():
+  """Host inputs of _normcons_setup: seeded parameters (numpy, the draws
+  of pw.param_init) and (crystal, pseudopotential, G grid, mask, k-points,
+  occupation)."""
+  import numpy as np
+  from jrystal_amd import grid, occupation, pseudopotential as PS
+  from jrystal_amd.crystal import Crystal
+  here = os.path.dirname(os.path.abspath(__file__))
+  cr = Crystal.create_from_file(os.path.join(here, "jrystal_amd", "geometry",
+                                             "diamond.xyz"))
+  pp = PS.NormConservingPseudopotential.create(cr)
+  gs = grid.proper_grid_size(48)
+  g = np.asarray(grid.g_vectors(cr.cell_vectors, gs))
+  mask = np.asarray(grid.spherical_mask(cr.cell_vectors, gs, 100.0))
+  kpts = np.asarray(grid.k_vectors(cr.cell_vectors, grid.proper_grid_size(2)))
+  nb = 14
+  occ = occupation.uniform(kpts.shape[0], pp.total_valence_charge, 0, nb)
+  occ = np.asarray(occ.cpu() if hasattr(occ, "cpu") else occ, np.float64)
+  rng = np.random.default_rng(0)
+  shape = (1, kpts.shape[0], int(mask.sum()), nb)
+  params = {"w_re": rng.random(shape), "w_im": rng.random(shape)}
+  return params, (cr, pp, g, mask, kpts, occ)
+
+
+def normcons_bench(steps=200, warmup=10):
+  """Energy + gradient evaluations of the norm-conserving diamond example
+  (local pseudopotential, non-local projectors), eager, with the per-class
+  times of one evaluation."""
+  import torch
+  plan, params, _ = _normcons_setup()
+  params = {k: torch.tensor(v, device=plan.device) for k, v in params.items()}
+  out = {}
+  print(f"bench: norm-conserving {NC_WORKLOAD}", file=sys.stderr, flush=True)
+  for _ in range(warmup):
+    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+  torch.cuda.synchronize()
+  dt = (time.perf_counter() - t0) / steps
+  plan.set_timing(True)
+  plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+  torch.cuda.synchronize()
+  cls = {k: round(v[0], 4) for k, v in plan.read_timing().items() if v[1]}
+  plan.set_timing(False)
+  e = out["energies"].tolist()
+  res = {"workload": NC_WORKLOAD, "num_g": int(plan.num_g),
+         "num_kpts": int(plan.nk), "num_bands": int(plan.nb),
+         "projectors": int(plan.num_projectors), "steps": steps,
+         "ms_per_eval": dt * 1e3, "evals_per_s": 1.0 / dt,
+         "class_ms_one_eval": cls, "energy_total_Ha": e[4]}
+  plan.close()
+  torch.cuda.empty_cache()
+  return res
+
+
+def normcons_probe(evals=20):
+  """The workload of the norm-conserving rocprofv3 runs."""
+  import torch
+  plan, params, _ = _normcons_setup()
+  params = {k: torch.tensor(v, device=plan.device) for k, v in params.items()}
+  out = {}
+  for _ in range(evals):
+    plan.evaluate(params["w_re"], params["w_im"], grad=True, out=out)
+  torch.cuda.synchronize()
+  print(json.dumps({"probe": NC_WORKLOAD, "evaluations": evals,
+                    "energy_total_Ha": float(out["energies"][4])}))
+
+
+def cpu_normcons(target_s, nthreads):
+  """The oracle's norm-conserving evaluation of the diamond example on a
+  k-point subset, scaled to whole evaluations per second (projector
+  tables built before the timed region)."""
+  sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+  import numpy as np
+  from threadpoolctl import threadpool_limits
+  from oracle import adjoint as OA
+  from oracle import pseudopotential as OPP
+  from oracle import pw as OP
+  OP.set_workers(nthreads)
+  limits = threadpool_limits(limits=nthreads)
+  params, (cr, pp, g, mask, kpts, occ) = _normcons_inputs()
+  opp = OPP.create(cr.symbols, cr.positions, os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "jrystal_amd", "data",
+    "pseudopotential", "normconserving"))
+  vloc = OPP.local_potential_reciprocal(opp, np.asarray(g), cr.vol)
+  nk_full = kpts.shape[0]
+  p_np = params
+
+  def run(nk_s):
+    kp = np.asarray(kpts)[:nk_s]
+    prob = OA.Problem(mask=np.asarray(mask), g_vector_grid=np.asarray(g),
+                      kpts=kp, vol=cr.vol, occupation=occ[:, :nk_s],
+                      positions=cr.positions,
+                      charges=np.asarray(cr.charges, float), v_ext_rec=vloc,
+                      projectors=OPP.projectors(opp, np.asarray(g), mask, kp))
+    p = {k: v[:, :nk_s] for k, v in p_np.items()}
+    t = time.perf_counter()
+    OA.energy_and_grad(p, prob)
+    return time.perf_counter() - t
+
+  run(1)
+  t1 = run(1)
+  nk_s = int(max(1, min(nk_full, round(target_s / max(t1, 1e-3)))))
+  t = run(nk_s)
+  limits.unregister()
+  return {"value": nk_s / (t * nk_full),
+          "unit": "energy+gradient evaluations/s", "cores": nthreads,
+          "kind": "port",
+          "sample": (f"the norm-conserving diamond example restricted to "
+                     f"{nk_s} of {nk_full} k-points ({t:.2f} s), scaled by "
+                     f"{nk_full}/{nk_s}; oracle/ NumPy+SciPy FP64")}
+
+
 def band_probe(steps=100):
   """The workload of the band-mode rocprofv3 runs: ``steps`` Adam steps of
   the band driver's objective at the first k-point of the Si band example
@@ -711,6 +851,9 @@ def main():
   ap.add_argument("--no-band-structure", action="store_true")
   ap.add_argument("--band-probe", action="store_true",
                   help="only the band-mode rocprofv3 workload")
+  ap.add_argument("--no-normcons", action="store_true")
+  ap.add_argument("--normcons-probe", action="store_true",
+                  help="only the norm-conserving rocprofv3 workload")
   ap.add_argument("--no-large-cell", action="store_true")
   ap.add_argument("--large-cell-steps", type=int, default=5)
   ap.add_argument("--large-cell-probe", action="store_true",
@@ -721,6 +864,9 @@ def main():
     return
   if args.band_probe:
     band_probe()
+    return
+  if args.normcons_probe:
+    normcons_probe()
     return
   if args.large_cell_probe:
     large_cell_probe()
@@ -832,6 +978,10 @@ def main():
   if not args.no_band_structure:
     bands = band_structure_bench()
 
+  nc = None
+  if not args.no_normcons:
+    nc = normcons_bench()
+
   conv = None
   if not args.no_convergence:
     conv, conv_ctx = convergence(args.conv_seconds)
@@ -848,6 +998,8 @@ def main():
     if bands is not None:
       cpu["band_structure_step"] = cpu_band_step(args.cpu_seconds / 2,
                                                  nthreads)
+    if nc is not None:
+      cpu["normcons"] = cpu_normcons(args.cpu_seconds / 2, nthreads)
     if conv is not None:
       # CPU seconds per evaluation of the convergence cell, times the
       # evaluations the plain method needed on the GPU
@@ -879,6 +1031,7 @@ def main():
     "small_cell": small,
     "large_cell": large,
     "band_structure": bands,
+    "normcons": nc,
   }
   print(json.dumps(line))
 

@@ -86,10 +86,18 @@ __global__ void nl_reduce_kernel(const double2* __restrict__ part,
   if (e >= P * npairs) return;
   const int64_t p = e / npairs;
   const int pair = (int)(e - p * npairs);
-  double2 a = czero();
-  for (int c = 0; c < nchunk; ++c)
-    a = cadd(a, part[(p * nchunk + c) * npairs + pair]);
-  F[e] = a;
+  // four sums in flight, added in a fixed order
+  const double2* src = part + p * nchunk * npairs + pair;
+  double2 a = czero(), b = czero(), c2 = czero(), d = czero();
+  int c = 0;
+  for (; c + 3 < nchunk; c += 4) {
+    a = cadd(a, src[(int64_t)c * npairs]);
+    b = cadd(b, src[(int64_t)(c + 1) * npairs]);
+    c2 = cadd(c2, src[(int64_t)(c + 2) * npairs]);
+    d = cadd(d, src[(int64_t)(c + 3) * npairs]);
+  }
+  for (; c < nchunk; ++c) a = cadd(a, src[(int64_t)c * npairs]);
+  F[e] = cadd(cadd(a, b), cadd(c2, d));
 }
 
 // e[p][b] = sum_j |F[p][j][b]|^2 / vol
@@ -159,8 +167,10 @@ __global__ __launch_bounds__(256) void nl_finalize_kernel(
 }  // namespace
 
 int nl_rows(int ng, int64_t P) {
+  // about 256 blocks or more: enough to spread the projections, few enough
+  // partials for the chunk sum
   int rows = 1024;
-  while (rows > kNlTile && (int64_t)ceil_div(ng, rows) * P < 1024) rows >>= 1;
+  while (rows > kNlTile && (int64_t)ceil_div(ng, rows) * P < 256) rows >>= 1;
   return rows;
 }
 
