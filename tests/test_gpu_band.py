@@ -264,3 +264,24 @@ def test_band_driver_cg_reaches_dense_eigenvalues(tmp_path):
     err = np.max(np.abs(out.eigenvalues[0, i] - lam))
     print(f"k{i}: max |eig - dense| = {err:.2e}", flush=True)
     assert err <= 1e-6 * max(1.0, np.max(np.abs(lam)))
+
+
+@pytest.mark.parametrize("nb", [40, 70])
+def test_subspace_eigenvalues_many_bands(nb):
+  """The whole-block Jacobi beyond one register tile (40 bands: LDS; 70
+  bands: matrices in global scratch) and the wide QR in band mode."""
+  from jrystal_amd import workloads as W
+  wl = W.build(f"si2-16^3-{nb}b", W.fcc_si2(4.0), (16, 16, 16), (1, 1, 1),
+               nb, 15.0)
+  assert wl.num_g > nb
+  prob = _problem(wl)
+  rho = _density(prob, 21)
+  v = OB.effective_potential(rho, prob)
+  ones = np.ones_like(prob.occupation)
+  plan = _band_plan(wl, prob, rho, occ=ones)
+  p = seeded_params(wl.shape, 22)
+  _eval(plan, p, grad=False)
+  ev = plan.subspace_eigenvalues().cpu().numpy()
+  prob1 = dataclasses.replace(prob, occupation=ones)
+  eref = OB.band_eigenvalues(p, prob1, v)
+  assert rel_max(ev, eref) <= 1e-9
