@@ -51,7 +51,12 @@ def xc_energy(density_grid, g_vector_grid, vol, xc_type="lda_x",
   formed as sigma_r_fn forms them)."""
   assert density_grid.ndim == 4, "density_grid must contains spin axis"
   if kohn_sham:
-    raise NotImplementedError("kohn_sham xc potentials are not on this path")
+    # xc_density(kohn_sham=True) is vxc_lda (xc.py:109-124, 247-250): for
+    # one channel of LDA exchange d(rho eps)/d rho = 4/3 eps pointwise, so
+    # the sum of v rho is 4/3 of the exchange energy
+    if xc_type != "lda_x" or density_grid.shape[0] != 1:
+      raise NotImplementedError("kohn_sham xc energy: unpolarised lda_x only")
+    return (4.0 / 3.0) * _ops.ExchangeLDA.apply(density_grid, float(vol))
   if xc_type == "lda_x":
     return _ops.ExchangeLDA.apply(density_grid, float(vol))
   from .objective import xc_terms

@@ -285,3 +285,30 @@ def test_subspace_eigenvalues_many_bands(nb):
   prob1 = dataclasses.replace(prob, occupation=ones)
   eref = OB.band_eigenvalues(p, prob1, v)
   assert rel_max(ev, eref) <= 1e-9
+
+
+def test_trace_equals_kohn_sham_total_energy():
+  """hamiltonian_test.py:78-102 through the public functions: the band
+  trace at the density of the same orbitals (all-ones occupation) equals
+  energy.total_energy(kohn_sham=True) (atol 1e-7 there, 1e-9 here)."""
+  import os
+  from jrystal_amd import energy, grid, pw
+  from jrystal_amd.crystal import Crystal
+  from jrystal_amd.hamiltonian import band_plan, hamiltonian_matrix_trace
+  here = os.path.dirname(os.path.abspath(__file__))
+  cr = Crystal.create_from_file(os.path.join(here, "golden", "diamond.xyz"))
+  kpts = grid.k_vectors(cr.cell_vectors, [2, 2, 1])
+  g = grid.g_vectors(cr.cell_vectors, [7, 8, 9])
+  mask = grid.cubic_mask([7, 8, 9])
+  nb = cr.num_electron
+  params = pw.param_init(123, nb, kpts.shape[0], mask)
+  coeff = pw.coeff(params, mask)
+  occ = np.ones((1, kpts.shape[0], nb))
+  rho = pw.density_grid(coeff, cr.vol, occ)
+  plan = band_plan(cr.positions, cr.charges, cr.cell_vectors, mask, rho,
+                   kpts, nb)
+  e1 = float(hamiltonian_matrix_trace(plan, params["w_re"], params["w_im"]))
+  e2 = float(energy.total_energy(coeff, cr.positions, cr.charges, g, kpts,
+                                 cr.vol, kohn_sham=True))
+  print(f"trace {e1:.12f} kohn-sham total {e2:.12f}", flush=True)
+  assert abs(e1 - e2) < 1e-9 * max(1.0, abs(e2))
