@@ -21,20 +21,28 @@ constexpr int kNlBlock = 256;
 constexpr int kNlTile = 32;      // g per LDS tile
 constexpr int kNlPairs = 2048;   // (j, b) pairs of one block (8 per thread)
 
-// Partial F over the g-range of chunk c: part[p][c][pair], pair = j nb + b
-// within this block's pair range [pr0, pr0 + kNlPairs).
+// Partial F over the g-range of chunk c: part[p][c][pair], pair = j nb + b,
+// for the npb pairs of this block starting at pr0 = blockIdx.z kNlPairs.
+// Thread t serves the pairs t / S + (kNlBlock / S) r and, of every tile of
+// 32 g, the g with g % S == t % S.  S > 1 only when the block has at most
+// kNlBlock / S pairs (one round), so that few pairs still occupy every
+// thread; the S slice sums of a pair are then added in slice order.
+// Dynamic LDS: the projector tile [32][33] and the Q tile [nb][33].
 __global__ __launch_bounds__(kNlBlock) void nl_project_kernel(
     const double2* __restrict__ proj, const double2* __restrict__ q, int nk,
-    int np_, int ng, int nb, int rows, int nchunk,
+    int np_, int ng, int nb, int rows, int nchunk, int S,
     double2* __restrict__ part) {
-  __shared__ double2 sp[kNlTile][kNlTile + 1];   // proj tile [j][g], 32 j
-  __shared__ double2 sq[128][kNlTile + 1];       // Q tile [b][g]
+  extern __shared__ double2 nl_lds[];
+  double2* sp = nl_lds;                                // [32][33]
+  double2* sq = nl_lds + kNlTile * (kNlTile + 1);      // [nb][33]
   const int c = blockIdx.x;
   const int64_t p = blockIdx.y;
   const int pr0 = blockIdx.z * kNlPairs;
   const int k = (int)(p % nk);
   const int npairs = np_ * nb;
+  const int npb = min(kNlPairs, npairs - pr0);
   const int tid = threadIdx.x;
+  const int slots = kNlBlock / S, ps = tid / S, sl = tid - ps * S;
   const int g0 = c * rows, g1 = min(ng, g0 + rows);
   const double2* P = proj + (int64_t)k * np_ * ng;
   const double2* Qp = q + p * nb * ng;
@@ -48,37 +56,52 @@ __global__ __launch_bounds__(kNlBlock) void nl_project_kernel(
       __syncthreads();
       for (int e = tid; e < kNlTile * kNlTile; e += kNlBlock) {
         const int j = e / kNlTile, g = e % kNlTile;
-        sp[j][g] = (j < jn && g < w) ? P[(int64_t)(jt + j) * ng + gt + g]
-                                     : czero();
+        sp[j * (kNlTile + 1) + g] =
+            (j < jn && g < w) ? P[(int64_t)(jt + j) * ng + gt + g] : czero();
       }
       if (jt == 0)
         for (int e = tid; e < nb * kNlTile; e += kNlBlock) {
           const int b = e / kNlTile, g = e % kNlTile;
-          sq[b][g] = g < w ? Qp[(int64_t)b * ng + gt + g] : czero();
+          sq[b * (kNlTile + 1) + g] =
+              g < w ? Qp[(int64_t)b * ng + gt + g] : czero();
         }
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < kNlPairs / kNlBlock; ++i) {
-        const int pair = pr0 + tid + i * kNlBlock;
-        if (pair >= npairs) continue;
+        const int lp = ps + i * slots;
+        if (lp >= npb) continue;
+        const int pair = pr0 + lp;
         const int j = pair / nb, b = pair - j * nb;
         if (j < jt || j >= jt + jn) continue;
+        const double2* a_row = sp + (j - jt) * (kNlTile + 1);
+        const double2* b_row = sq + b * (kNlTile + 1);
         double2 a = acc[i];
-        for (int g = 0; g < w; ++g) a = cadd(a, cmul(sp[j - jt][g], sq[b][g]));
+        for (int g = sl; g < w; g += S) a = cadd(a, cmul(a_row[g], b_row[g]));
         acc[i] = a;
       }
     }
   }
-  double2* out = part + (p * nchunk + c) * npairs;
+  double2* out = part + (p * nchunk + c) * npairs + pr0;
+  if (S == 1) {
 #pragma unroll
-  for (int i = 0; i < kNlPairs / kNlBlock; ++i) {
-    const int pair = pr0 + tid + i * kNlBlock;
-    if (pair < npairs) out[pair] = acc[i];
+    for (int i = 0; i < kNlPairs / kNlBlock; ++i) {
+      const int lp = tid + i * kNlBlock;
+      if (lp < npb) out[lp] = acc[i];
+    }
+    return;
+  }
+  __syncthreads();  // the tiles are no longer read: reuse them
+  nl_lds[tid] = acc[0];
+  __syncthreads();
+  if (sl == 0 && ps < npb) {
+    double2 a = nl_lds[tid];
+    for (int t = 1; t < S; ++t) a = cadd(a, nl_lds[tid + t]);
+    out[ps] = a;
   }
 }
 
-// F[p][pair] = sum_c part[p][c][pair] (fixed order); e[p][b] when the
-// whole band column is in one thread's reach is formed by nl_energy.
+// F[p][pair] = sum_c part[p][c][pair] (fixed order): one thread per output
+// with four sums in flight, for few chunks.
 __global__ void nl_reduce_kernel(const double2* __restrict__ part,
                                  int64_t P, int nchunk, int npairs,
                                  double2* __restrict__ F) {
@@ -86,7 +109,6 @@ __global__ void nl_reduce_kernel(const double2* __restrict__ part,
   if (e >= P * npairs) return;
   const int64_t p = e / npairs;
   const int pair = (int)(e - p * npairs);
-  // four sums in flight, added in a fixed order
   const double2* src = part + p * nchunk * npairs + pair;
   double2 a = czero(), b = czero(), c2 = czero(), d = czero();
   int c = 0;
@@ -98,6 +120,28 @@ __global__ void nl_reduce_kernel(const double2* __restrict__ part,
   }
   for (; c < nchunk; ++c) a = cadd(a, src[(int64_t)c * npairs]);
   F[e] = cadd(cadd(a, b), cadd(c2, d));
+}
+
+// The same sum with one wave per output, for many chunks: lane l adds the
+// chunks l, l + 64, ... in order, then the lanes are combined by a fixed
+// butterfly.  Four waves per block; a wave past the end returns whole.
+__global__ __launch_bounds__(256) void nl_reduce_wave_kernel(
+    const double2* __restrict__ part, int64_t P, int nchunk, int npairs,
+    double2* __restrict__ F) {
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (e >= P * npairs) return;
+  const int64_t p = e / npairs;
+  const int pair = (int)(e - p * npairs);
+  const double2* src = part + p * nchunk * npairs + pair;
+  double2 a = czero();
+  for (int c = lane; c < nchunk; c += 64) a = cadd(a, src[(int64_t)c * npairs]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a.x += __shfl_xor(a.x, off);
+    a.y += __shfl_xor(a.y, off);
+  }
+  if (lane == 0) F[e] = a;
 }
 
 // e[p][b] = sum_j |F[p][j][b]|^2 / vol
@@ -193,15 +237,26 @@ int launch_nonlocal(const double2* proj, const double2* q, const double* occ,
   const int npairs = np_ * nb;
   double2* part = static_cast<double2*>(scratch);
   double2* F = part + (size_t)P * nchunk * npairs;
+  // g slices per pair: the threads of a block not needed for its pairs
+  const int npb = npairs < kNlPairs ? npairs : kNlPairs;
+  int S = 1;
+  while (S < kNlTile && npb * S * 2 <= kNlBlock) S *= 2;
+  const size_t shm =
+      (size_t)(kNlTile + nb) * (kNlTile + 1) * sizeof(double2);
   hipLaunchKernelGGL(nl_project_kernel,
                      dim3((unsigned)nchunk, (unsigned)P,
                           (unsigned)ceil_div(npairs, kNlPairs)),
-                     dim3(kNlBlock), 0, s, proj, q, nk, np_, ng, nb, rows,
-                     nchunk, part);
+                     dim3(kNlBlock), shm, s, proj, q, nk, np_, ng, nb, rows,
+                     nchunk, S, part);
   JRX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(nl_reduce_kernel,
-                     dim3((unsigned)ceil_div(P * npairs, (int64_t)256)),
-                     dim3(256), 0, s, part, P, nchunk, npairs, F);
+  if (nchunk > 16)
+    hipLaunchKernelGGL(nl_reduce_wave_kernel,
+                       dim3((unsigned)ceil_div(P * npairs, (int64_t)4)),
+                       dim3(256), 0, s, part, P, nchunk, npairs, F);
+  else
+    hipLaunchKernelGGL(nl_reduce_kernel,
+                       dim3((unsigned)ceil_div(P * npairs, (int64_t)256)),
+                       dim3(256), 0, s, part, P, nchunk, npairs, F);
   JRX_LAUNCH_CHECK();
   hipLaunchKernelGGL(nl_energy_kernel,
                      dim3((unsigned)ceil_div(P * nb, (int64_t)256)),

@@ -33,8 +33,9 @@ def _case(geom, gs, ecut, kgrid, nb, seed=0):
   from jrystal_amd import pseudopotential as PS
   from jrystal_amd.crystal import Crystal
   from jrystal_amd.objective import EnergyPlan
-  cr = Crystal.create_from_file(os.path.join(GEOM, geom + ".xyz"))
-  ocr = OCrystal.create_from_file(os.path.join(GEOM, geom + ".xyz"))
+  path = geom if geom.endswith(".xyz") else os.path.join(GEOM, geom + ".xyz")
+  cr = Crystal.create_from_file(path)
+  ocr = OCrystal.create_from_file(path)
   g = OG.g_vectors(ocr.cell_vectors, list(gs))
   mask = OG.spherical_mask(ocr.cell_vectors, list(gs), ecut)
   kpts = OG.k_vectors(ocr.cell_vectors, list(kgrid))
@@ -90,6 +91,34 @@ def test_normcons_energy_density_gradient(geom, gs, ecut, kgrid, nb):
   _eval(plan, p, density=False)
   be = plan.band_energies().cpu().numpy()
   assert rel_max(be, OA.band_energies(p, prob)) <= 1e-9
+
+
+def test_normcons_many_projectors_two_pair_blocks(tmp_path):
+  """Si16 (si8 doubled along x): 64 projectors (two tiles of 32) x 40 bands
+  = 2560 (projector, band) pairs, two pair blocks of the projection with
+  one g slice each, and 27 chunks of 32 g (855 coefficients, one k-point)
+  summed by the one-wave-per-output reduction (more than 16 chunks)."""
+  lines = open(os.path.join(GEOM, "si8.xyz")).read().splitlines()
+  a = float(lines[1].split('"')[1].split()[0])
+  atoms = [ln.split() for ln in lines[2:] if ln.strip()]
+  out = [str(2 * len(atoms)),
+         f'Lattice="{2 * a} 0.0 0.0 0.0 {a} 0.0 0.0 0.0 {a}" '
+         'Properties=species:S:1:pos:R:3 pbc="T T T"']
+  for shift in (0.0, a):
+    out += [f"{s} {float(x) + shift} {y} {z}" for s, x, y, z in atoms]
+  geom = str(tmp_path / "si16.xyz")
+  open(geom, "w").write("\n".join(out) + "\n")
+  plan, prob, p = _case(geom, (32, 16, 16), 4.0, (1, 1, 1), 40)
+  assert plan.num_projectors == 64 and int(prob.mask.sum()) > 16 * 32
+  res = _eval(plan, p)
+  parts, grad, rho = OA.energy_and_grad(p, prob)
+  print("si16 external_nonlocal", parts["external_nonlocal"], flush=True)
+  for i, k in enumerate(NAMES):
+    assert abs(res["energies"][i] - parts[k]) <= 1e-8, (k, res["energies"][i],
+                                                         parts[k])
+  assert np.max(np.abs(res["density"] - rho)) <= 1e-10
+  assert rel_max(res["dw_re"], grad["w_re"]) <= 1e-7
+  assert rel_max(res["dw_im"], grad["w_im"]) <= 1e-7
 
 
 def test_switching_off_restores_all_electron():
