@@ -705,7 +705,8 @@ int launch_gram_gmajor(const double* wr, const double* wi, const double2* pre,
 }
 
 // Partial sums of G = Q^H Qbar over the chunk (band-major inputs), one per
-// block.
+// block.  A wave takes 16-row slabs in turn, 4 rows at a time (row
+// g0 + 4 r + lk, bands 16 I + li).
 template <int T>
 __global__ __launch_bounds__(kQrBlock) void gram_q_kernel(
     const double2* __restrict__ q, const double2* __restrict__ qb,
@@ -724,27 +725,57 @@ __global__ __launch_bounds__(kQrBlock) void gram_q_kernel(
   const int ge = min(ng, gb + rows_per_chunk);
   const double2* Q = q + p * nb * ng;
   const double2* B = qb + p * nb * ng;
-  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
+  auto accum = [&](const double2* a, const double2* b) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int g = g0 + 4 * r + lk;
-      double2 a[T], b[T];
+    for (int I = 0; I < T; ++I)
+#pragma unroll
+      for (int J = 0; J < T; ++J) {
+        gre[I][J] = mfma4(a[I].x, b[J].x, gre[I][J]);
+        gre[I][J] = mfma4(a[I].y, b[J].y, gre[I][J]);
+        gim[I][J] = mfma4(a[I].x, b[J].y, gim[I][J]);
+        gim[I][J] = mfma4(-a[I].y, b[J].x, gim[I][J]);
+      }
+  };
+  if constexpr (T <= 2) {
+    // units of 4 rows; the loads of the next unit are issued before the
+    // MFMAs of the current one
+    const int gw = gb + wave * 16;  // first slab of this wave
+    const int nunit = gw < ge ? ((ge - gw + 63) / 64) * 4 : 0;
+    double2 ca[T], cb[T];
+    auto load = [&](int u, double2* a, double2* b) {
+      const int g = gw + 64 * (u >> 2) + 4 * (u & 3) + lk;
 #pragma unroll
       for (int I = 0; I < T; ++I) {
         const int band = 16 * I + li;
-        const bool ok = g < ge && band < nb;
+        const bool ok = u < nunit && g < ge && band < nb;
         a[I] = ok ? Q[(int64_t)band * ng + g] : czero();
         b[I] = ok ? B[(int64_t)band * ng + g] : czero();
       }
+    };
+    load(0, ca, cb);
+    for (int u = 0; u < nunit; ++u) {
+      double2 na[T], nbv[T];
+      load(u + 1, na, nbv);
+      accum(ca, cb);
 #pragma unroll
-      for (int I = 0; I < T; ++I)
+      for (int I = 0; I < T; ++I) { ca[I] = na[I]; cb[I] = nbv[I]; }
+    }
+  } else {
+    // whole slabs (more registers per unit than a second buffer allows)
+    for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
 #pragma unroll
-        for (int J = 0; J < T; ++J) {
-          gre[I][J] = mfma4(a[I].x, b[J].x, gre[I][J]);
-          gre[I][J] = mfma4(a[I].y, b[J].y, gre[I][J]);
-          gim[I][J] = mfma4(a[I].x, b[J].y, gim[I][J]);
-          gim[I][J] = mfma4(-a[I].y, b[J].x, gim[I][J]);
+      for (int r = 0; r < 4; ++r) {
+        const int g = g0 + 4 * r + lk;
+        double2 a[T], b[T];
+#pragma unroll
+        for (int I = 0; I < T; ++I) {
+          const int band = 16 * I + li;
+          const bool ok = g < ge && band < nb;
+          a[I] = ok ? Q[(int64_t)band * ng + g] : czero();
+          b[I] = ok ? B[(int64_t)band * ng + g] : czero();
         }
+        accum(a, b);
+      }
     }
   }
   // Sum the four waves' tiles in LDS (wave order 0..3), one partial per
@@ -1124,7 +1155,7 @@ __global__ __launch_bounds__(kQrBlock) void qr_bwd_small_kernel(
   for (int e = tid; e < nb * nb; e += kQrBlock) {
     const int i = e / nb, j = e - i * nb;
     // (R^-H)[k][j] = conj(Ri[j][k]), nonzero for k >= j
-    Ap[e] = cconj(Ri[j * ld + i]);
+    Ap[e] = i >= j ? cconj(Ri[j * ld + i]) : czero();
     double2 acc = czero();
     for (int k = j; k < nb; ++k) acc = cadd(acc, cmul(M[i * ld + k], cconj(Ri[j * ld + k])));
     Bp[e] = acc;
@@ -1158,7 +1189,10 @@ inline int apply_rows(int ng, int64_t P) {
 }
 
 // Q[p][b][g] = sum_i W[p][g][i] rinv[p][i][b], computed as Q^T = R^T W^T so
-// that each accumulator row is 16 consecutive g of one band.
+// that each accumulator row is 16 consecutive g of one band.  rinv is upper
+// triangular (as the Cholesky inverses and their products are), so up to
+// 48 bands the (column group, band tile) pairs below the diagonal are
+// skipped.  A wave takes 16-row slabs in turn.
 template <int T>
 __global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
     const double* __restrict__ wr, const double* __restrict__ wi,
@@ -1176,34 +1210,93 @@ __global__ __launch_bounds__(kQrBlock) void apply_fwd_kernel(
   double2* Q = q + p * nb * ng;
   const int gb = blockIdx.x * rows;
   const int ge = min(ng, gb + rows);
-  const int nm = (nb + 3) / 4;
-  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
+  const int gw = gb + wave * 16;  // first slab of this wave
+  if constexpr (T <= 3) {
+    // units of 16 columns; the loads of the next unit are issued before
+    // the MFMAs of the current one
+    const int nunit = gw < ge ? ((ge - gw + 63) / 64) * T : 0;
+    double ar[4], ai[4];
+    auto load = [&](int u, double* xr, double* xi) {
+      const int row = gw + 64 * (u / T) + li, mg = u % T;
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm) {
+        const int col = 16 * mg + 4 * mm + lk;
+        const bool ok = u < nunit && row < ge && col < nb;
+        xr[mm] = ok ? Wr[(int64_t)row * nb + col] : 0.0;
+        xi[mm] = ok ? Wi[(int64_t)row * nb + col] : 0.0;
+      }
+    };
+    load(0, ar, ai);
     d4 qre[T], qim[T];
 #pragma unroll
     for (int J = 0; J < T; ++J) { qre[J] = d4zero(); qim[J] = d4zero(); }
-    const int row = g0 + li;
-    for (int m = 0; m < nm; ++m) {
-      const int col = 4 * m + lk;
-      const bool ok = row < ge && col < nb;
-      const double a_r = ok ? Wr[(int64_t)row * nb + col] : 0.0;
-      const double a_i = ok ? Wi[(int64_t)row * nb + col] : 0.0;
+    for (int u = 0; u < nunit; ++u) {
+      double nr[4], ni[4];
+      load(u + 1, nr, ni);
+      const int mg = u % T;
 #pragma unroll
-      for (int J = 0; J < T; ++J) {
-        const double2 rv = sm[(4 * m + lk) * NBP + 16 * J + li];
-        qre[J] = mfma4(rv.x, a_r, qre[J]);
-        qre[J] = mfma4(-rv.y, a_i, qre[J]);
-        qim[J] = mfma4(rv.x, a_i, qim[J]);
-        qim[J] = mfma4(rv.y, a_r, qim[J]);
+      for (int mm = 0; mm < 4; ++mm) {
+        const int k = 16 * mg + 4 * mm + lk;
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+          if (mg > J) continue;  // rinv[i][b] = 0 for i > b
+          const double2 rv = sm[k * NBP + 16 * J + li];
+          qre[J] = mfma4(rv.x, ar[mm], qre[J]);
+          qre[J] = mfma4(-rv.y, ai[mm], qre[J]);
+          qim[J] = mfma4(rv.x, ai[mm], qim[J]);
+          qim[J] = mfma4(rv.y, ar[mm], qim[J]);
+        }
       }
+      if (mg == T - 1) {
+        const int g = gw + 64 * (u / T) + li;
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int b = 16 * J + 4 * r + lk;
+            if (b < nb && g < ge)
+              Q[(int64_t)b * ng + g] = make_double2(qre[J][r], qim[J][r]);
+          }
+          qre[J] = d4zero();
+          qim[J] = d4zero();
+        }
+      }
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm) { ar[mm] = nr[mm]; ai[mm] = ni[mm]; }
     }
-    const int g = g0 + li;
+  } else {
+    // one column group of 4 at a time (more registers per unit than a
+    // second buffer allows)
+    const int nm = (nb + 3) / 4;
+    for (int g0 = gw; g0 < ge; g0 += 64) {
+      d4 qre[T], qim[T];
 #pragma unroll
-    for (int J = 0; J < T; ++J)
+      for (int J = 0; J < T; ++J) { qre[J] = d4zero(); qim[J] = d4zero(); }
+      const int row = g0 + li;
+      for (int m = 0; m < nm; ++m) {
+        const int col = 4 * m + lk;
+        const bool ok = row < ge && col < nb;
+        const double a_r = ok ? Wr[(int64_t)row * nb + col] : 0.0;
+        const double a_i = ok ? Wi[(int64_t)row * nb + col] : 0.0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int b = 16 * J + 4 * r + lk;
-        if (b < nb && g < ge) Q[(int64_t)b * ng + g] = make_double2(qre[J][r], qim[J][r]);
+        for (int J = 0; J < T; ++J) {
+          const double2 rv = sm[(4 * m + lk) * NBP + 16 * J + li];
+          qre[J] = mfma4(rv.x, a_r, qre[J]);
+          qre[J] = mfma4(-rv.y, a_i, qre[J]);
+          qim[J] = mfma4(rv.x, a_i, qim[J]);
+          qim[J] = mfma4(rv.y, a_r, qim[J]);
+        }
       }
+      const int g = g0 + li;
+#pragma unroll
+      for (int J = 0; J < T; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int b = 16 * J + 4 * r + lk;
+          if (b < nb && g < ge)
+            Q[(int64_t)b * ng + g] = make_double2(qre[J][r], qim[J][r]);
+        }
+    }
   }
 }
 
@@ -1247,7 +1340,10 @@ int launch_apply_fwd(const double* wr, const double* wi, const double2* rinv,
 }
 
 // Wbar[p][g][j] = sum_i Qbar[p][i][g] A[i][j] + Q[p][i][g] Bm[i][j], planar
-// g-major outputs.
+// g-major outputs.  A = R^{-H} is lower triangular, so its (band group,
+// column tile) pairs above the diagonal are skipped.  A wave takes 16-row
+// slabs in turn and reads each slab in T units of 16 bands; the loads of the
+// next unit are issued before the MFMAs of the current one.
 template <int T>
 __global__ __launch_bounds__(kQrBlock) void apply_bwd_kernel(
     const double2* __restrict__ qbar, const double2* __restrict__ q,
@@ -1270,41 +1366,65 @@ __global__ __launch_bounds__(kQrBlock) void apply_bwd_kernel(
   double* Di = dwi + p * ng * nb;
   const int gb = blockIdx.x * rows;
   const int ge = min(ng, gb + rows);
-  const int nm = (nb + 3) / 4;
-  for (int g0 = gb + wave * 16; g0 < ge; g0 += 64) {
-    d4 re[T], im[T];
+  const int gw = gb + wave * 16;  // first slab of this wave
+  const int nunit = gw < ge ? ((ge - gw + 63) / 64) * T : 0;
+  double2 cb[4], cq[4];
+  auto load = [&](int u, double2* vb, double2* vq) {
+    const int g = gw + 64 * (u / T) + li, mg = u % T;
 #pragma unroll
-    for (int J = 0; J < T; ++J) { re[J] = d4zero(); im[J] = d4zero(); }
-    const int g = g0 + li;
-    for (int m = 0; m < nm; ++m) {
-      const int band = 4 * m + lk;
-      const bool ok = g < ge && band < nb;
-      const double2 vb = ok ? Qb[(int64_t)band * ng + g] : czero();
-      const double2 vq = ok ? Q[(int64_t)band * ng + g] : czero();
+    for (int mm = 0; mm < 4; ++mm) {
+      const int band = 16 * mg + 4 * mm + lk;
+      const bool ok = u < nunit && g < ge && band < nb;
+      vb[mm] = ok ? Qb[(int64_t)band * ng + g] : czero();
+      vq[mm] = ok ? Q[(int64_t)band * ng + g] : czero();
+    }
+  };
+  load(0, cb, cq);
+  d4 re[T], im[T];
+#pragma unroll
+  for (int J = 0; J < T; ++J) { re[J] = d4zero(); im[J] = d4zero(); }
+  for (int u = 0; u < nunit; ++u) {
+    double2 nbv[4], nqv[4];
+    load(u + 1, nbv, nqv);
+    const int mg = u % T;
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) {
+      const int k = 16 * mg + 4 * mm + lk;
+      const double2 vb = cb[mm], vq = cq[mm];
 #pragma unroll
       for (int J = 0; J < T; ++J) {
-        const double2 a = sa[(4 * m + lk) * NBP + 16 * J + li];
-        const double2 b = sb[(4 * m + lk) * NBP + 16 * J + li];
-        re[J] = mfma4(vb.x, a.x, re[J]);
-        re[J] = mfma4(-vb.y, a.y, re[J]);
+        if (J <= mg) {  // A[i][j] = 0 for j > i
+          const double2 a = sa[k * NBP + 16 * J + li];
+          re[J] = mfma4(vb.x, a.x, re[J]);
+          re[J] = mfma4(-vb.y, a.y, re[J]);
+          im[J] = mfma4(vb.x, a.y, im[J]);
+          im[J] = mfma4(vb.y, a.x, im[J]);
+        }
+        const double2 b = sb[k * NBP + 16 * J + li];
         re[J] = mfma4(vq.x, b.x, re[J]);
         re[J] = mfma4(-vq.y, b.y, re[J]);
-        im[J] = mfma4(vb.x, a.y, im[J]);
-        im[J] = mfma4(vb.y, a.x, im[J]);
         im[J] = mfma4(vq.x, b.y, im[J]);
         im[J] = mfma4(vq.y, b.x, im[J]);
       }
     }
+    if (mg == T - 1) {
+      const int g0 = gw + 64 * (u / T);
 #pragma unroll
-    for (int J = 0; J < T; ++J)
+      for (int J = 0; J < T; ++J) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = g0 + 4 * r + lk, col = 16 * J + li;
-        if (row < ge && col < nb) {
-          Dr[(int64_t)row * nb + col] = re[J][r];
-          Di[(int64_t)row * nb + col] = im[J][r];
+        for (int r = 0; r < 4; ++r) {
+          const int row = g0 + 4 * r + lk, col = 16 * J + li;
+          if (row < ge && col < nb) {
+            Dr[(int64_t)row * nb + col] = re[J][r];
+            Di[(int64_t)row * nb + col] = im[J][r];
+          }
         }
+        re[J] = d4zero();
+        im[J] = d4zero();
       }
+    }
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) { cb[mm] = nbv[mm]; cq[mm] = nqv[mm]; }
   }
 }
 
