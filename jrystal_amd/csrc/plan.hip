@@ -1219,8 +1219,12 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R1, P, nb, s, one_pass));
   JRX_T(JRX_K_QR_GRAM, launch_gram_gmajor(w_re, w_im, R1, part, P, ng, nb, p->gsw, s, one_pass, q));
   JRX_T(JRX_K_QR_SMALL, launch_reduce_partials(part, Sm, P, p->gsw.nchunk, nb, s));
-  JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s, nullptr, one_pass));
-  JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));
+  if (nb <= 32) {  // Ri = R1^{-1} R2^{-1} formed by the factorisation
+    JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, Ri, P, nb, s, nullptr, one_pass, R1));
+  } else {
+    JRX_T(JRX_K_QR_SMALL, launch_chol_inv(Sm, R2, P, nb, s, nullptr, one_pass));
+    JRX_T(JRX_K_QR_SMALL, launch_trmm_uu(R1, R2, Ri, P, nb, s));
+  }
   JRX_T(JRX_K_QR_APPLY, launch_apply_fwd(w_re, w_im, Ri, q, P, ng, nb, s));
   // --- density -------------------------------------------------------------
   const int64_t SS = (int64_t)p->nk * nb;  // states per spin

@@ -994,19 +994,24 @@ __device__ __forceinline__ double rsqrt_newton(double a) {
   return a > 0.0 ? y : __builtin_nan("");
 }
 
+// With lmul (upper triangular, [P][nb][nb]) the output is lmul X instead
+// of X (the product of the two CholeskyQR2 inverses, summed over k in
+// ascending order as trmm_uu_kernel does).
 __global__ __launch_bounds__(kQrBlock) void chol_inv32_kernel(
     const double2* __restrict__ S, double2* __restrict__ rinv, int nb,
-    int* __restrict__ flag_out, const int* __restrict__ flag_in) {
+    int* __restrict__ flag_out, const int* __restrict__ flag_in,
+    const double2* __restrict__ lmul) {
   __shared__ double2 rowbuf[32][33];
   __shared__ double2 colbuf[32][33];
   __shared__ double red[kQrBlock / 64];
   const int64_t p = blockIdx.x;
   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
   double2* out = rinv + p * nb * nb;
-  if (flag_in && flag_in[p]) {  // uniform over the block
+  const double2* L = lmul ? lmul + p * nb * nb : nullptr;
+  if (flag_in && flag_in[p]) {  // uniform over the block: X = I
     for (int e = tid; e < nb * nb; e += kQrBlock) {
       const int i = e / nb, j = e - i * nb;
-      out[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+      out[e] = L ? L[e] : make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     return;
   }
@@ -1076,6 +1081,24 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv32_kernel(
     }
     __syncthreads();
   }
+  if (L) {  // X into LDS (the loop's last barrier has retired colbuf)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = rr[q >> 1], c = cc[q & 1];
+      colbuf[r][c] = (r < nb && c < nb && c >= r) ? X[q] : czero();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = rr[q >> 1], c = cc[q & 1];
+      if (r < nb && c < nb) {
+        double2 acc = czero();
+        for (int k = r; k <= c; ++k) acc = cadd(acc, cmul(L[r * nb + k], colbuf[k][c]));
+        out[r * nb + c] = acc;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int r = rr[q >> 1], c = cc[q & 1];
@@ -1084,7 +1107,12 @@ __global__ __launch_bounds__(kQrBlock) void chol_inv32_kernel(
 }
 
 int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
-                    hipStream_t s, int* flag_out, const int* flag_in) {
+                    hipStream_t s, int* flag_out, const int* flag_in,
+                    const double2* lmul) {
+  if (lmul && nb > 32) {
+    set_error("launch_chol_inv: lmul is for at most 32 bands");
+    return JRX_ERR_INVALID;
+  }
   if (nb > kNarrowBands) {
     const size_t shm = ((size_t)nb * (nb + 1) / 2 + 2 * nb) * sizeof(double2) +
                        (nb + kCholWideBlock / 64) * sizeof(double);
@@ -1096,7 +1124,7 @@ int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
   }
   if (nb <= 32) {
     hipLaunchKernelGGL(chol_inv32_kernel, dim3((unsigned)P), dim3(kQrBlock), 0,
-                       s, S, rinv, nb, flag_out, flag_in);
+                       s, S, rinv, nb, flag_out, flag_in, lmul);
     JRX_LAUNCH_CHECK();
     return JRX_OK;
   }

@@ -56,10 +56,13 @@ int launch_reduce_partials(const double2* partial, double2* out, int64_t P,
 // i.e. ||D S D - I||_F < kQrOnePass with D = diag(S)^-1/2, which bounds
 // cond(D S D) by (1 + x) / (1 - x) = 3 and cond(W D) by sqrt(3); else 0.
 // flag_in (optional): problems with flag_in[p] != 0 get R^{-1} = I.
+// lmul (optional, nb <= 32): rinv = lmul R^{-1} for upper-triangular lmul
+// [P][nb][nb] (trmm_uu fused into the factorisation).
 constexpr double kQrOnePass = 0.5;
 int launch_chol_inv(const double2* S, double2* rinv, int64_t P, int nb,
                     hipStream_t s, int* flag_out = nullptr,
-                    const int* flag_in = nullptr);
+                    const int* flag_in = nullptr,
+                    const double2* lmul = nullptr);
 
 // c = a * b for upper-triangular a, b.
 int launch_trmm_uu(const double2* a, const double2* b, double2* c, int64_t P,
