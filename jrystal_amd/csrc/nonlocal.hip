@@ -25,8 +25,9 @@ constexpr int kNlPairs = 2048;   // (j, b) pairs of one block (8 per thread)
 // for the npb pairs of this block starting at pr0 = blockIdx.z kNlPairs.
 // Thread t serves the pairs t / S + (kNlBlock / S) r and, of every tile of
 // 32 g, the g with g % S == t % S.  S > 1 only when the block has at most
-// kNlBlock / S pairs (one round), so that few pairs still occupy every
-// thread; the S slice sums of a pair are then added in slice order.
+// kNlBlock / S pairs (a single pass over them), so that few pairs still
+// occupy every thread; the S slice sums of a pair are then added in slice
+// order.
 // Dynamic LDS: the projector tile [32][33] and the Q tile [nb][33].
 __global__ __launch_bounds__(kNlBlock) void nl_project_kernel(
     const double2* __restrict__ proj, const double2* __restrict__ q, int nk,
