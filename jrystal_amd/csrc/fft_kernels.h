@@ -122,11 +122,12 @@ int row_chunks(const BoxGeom& geo);
 int zplane_groups(const BoxGeom& geo, int nstates);
 int launch_row_ifft(const double2* q, double2* R, int64_t state0, int nstates,
                     const BoxGeom& geo, const double2* tw, hipStream_t s);
-// rho_part[g][z][y][x] += sum_{c in group g} occ |psi_c|^2 / vol
+// rho_part[g][z][y][x] += sum_{c in group g} occ |psi_c|^2 / vol, or =
+// with store (then every group's plane is written, empty groups with zeros)
 int launch_zplane_density(const double2* R, double* rho_part, int groups,
                           const double* occ, int64_t state0, int nstates,
                           double inv_vol, const BoxGeom& geo,
-                          const double2* tw, hipStream_t s);
+                          const double2* tw, bool store, hipStream_t s);
 // R[c] <- rows of FFT_xy(v * IFFT_xy(R[c])) (in place), v [z][y][x]
 int launch_zplane_vpsi(double2* R, const double* v, int groups, int nstates,
                        const BoxGeom& geo, const double2* tw, hipStream_t s);

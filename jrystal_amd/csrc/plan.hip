@@ -1235,8 +1235,14 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   int nrec = 0, nexc = 0, nkin = 0;
   if (!fixed) {
     for (int sp = 0; sp < p->ns; ++sp) {
-      JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
       const int64_t step = p->keep_planes ? SS : p->chunk;
+      // on the row path the first chunk stores every group's plane when it
+      // runs all p->groups groups; otherwise the partials start from zero
+      const bool store_first =
+          p->rowpath &&
+          zplane_groups(p->geo, (int)(SS < step ? SS : step)) == p->groups;
+      if (!store_first)
+        JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
       for (int64_t s0 = 0; s0 < SS; s0 += step) {
         const int nst = (int)((SS - s0) < step ? (SS - s0) : step);
         const int64_t st0 = sp * SS + s0;
@@ -1247,7 +1253,8 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
           JRX_T(JRX_K_XLINE_DENSITY,
                 launch_zplane_density(Tc, rpart, zplane_groups(p->geo, nst),
                                       p->d_occ, st0, nst, 1.0 / p->vol,
-                                      p->geo, p->Lx.tw, s));
+                                      p->geo, p->Lx.tw,
+                                      store_first && s0 == 0, s));
         } else if (reg64) {
           JRX_T(JRX_K_PLANE_IFFT, launch_plane_ifft64(q, Tc, st0, nst, p->geo,
                                                       p->Lx.tw, s));

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(BLK, ZCfg<N>::WPE > 0 ? ZCfg<N>::WPE : 1) void
 zplane_density_kernel(
     const double2* __restrict__ R, double* __restrict__ rho_part,
     const double* __restrict__ occ, int64_t state0, int nstates,
-    double inv_vol, BoxGeom geo, const double2* __restrict__ tw) {
+    double inv_vol, BoxGeom geo, const double2* __restrict__ tw, int store) {
   using Z = ZPlane<N, CP, BLK>;
   constexpr int BLOCK = Z::BLOCK, V = Z::V, LPB = Z::LPB, YP = Z::YPD;
   extern __shared__ double2 sm[];  // P [N][kZRow], twiddles [N]
@@ -347,9 +347,18 @@ zplane_density_kernel(
   const int per = (nstates + G - 1) / G;
   const int cb = gi * per, ce = min(nstates, cb + per);
   const int y0 = blockIdx.z * (N / Z::YS);  // first line of this block
+  double* part = rho_part + ((int64_t)gi * N + z) * N * N;
   for (int i = tid; i < N; i += BLOCK) stw[i] = tw[i];
   __syncthreads();
-  if (cb >= ce) return;  // uniform over the block
+  if (cb >= ce) {  // uniform over the block: an empty group
+    if (store)
+#pragma unroll
+      for (int p = 0; p < YP; ++p)
+#pragma unroll
+        for (int t = 0; t < V; ++t)
+          part[(y0 + ln + p * LPB) * N + j + 8 * t] = 0.0;
+    return;
+  }
   typename Z::Packed ri[CP], xo;
   Z::row_table(ri, geo, ln, j);
   Z::x_table(xo, geo, j);
@@ -393,12 +402,15 @@ zplane_density_kernel(
     }
     __syncthreads();
   }
-  double* part = rho_part + ((int64_t)gi * N + z) * N * N;
+  // the group's plane: stored by the first sweep chunk, added to after
 #pragma unroll
   for (int p = 0; p < YP; ++p) {
     const int y = y0 + ln + p * LPB;
 #pragma unroll
-    for (int t = 0; t < V; ++t) part[y * N + j + 8 * t] += acc[p][t] * inv_vol;
+    for (int t = 0; t < V; ++t) {
+      double* dst = part + y * N + j + 8 * t;
+      *dst = store ? acc[p][t] * inv_vol : *dst + acc[p][t] * inv_vol;
+    }
   }
 }
 
@@ -594,12 +606,12 @@ template <int N, int CP, int BLK = ZCfg<N>::BLOCK>
 static int zdens_n(const double2* R, double* rho_part, int groups,
                    const double* occ, int64_t state0, int nstates,
                    double inv_vol, const BoxGeom& geo, const double2* tw,
-                   hipStream_t s) {
+                   bool store, hipStream_t s) {
   const size_t shm = (size_t)(N * kZRow + N) * sizeof(double2);
   hipLaunchKernelGGL((zplane_density_kernel<N, CP, BLK>),
                      dim3(N, groups, ZPlane<N, CP, BLK>::YS),
                      dim3(BLK), shm, s, R, rho_part, occ, state0,
-                     nstates, inv_vol, geo, tw);
+                     nstates, inv_vol, geo, tw, store ? 1 : 0);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }
@@ -607,22 +619,22 @@ static int zdens_n(const double2* R, double* rho_part, int groups,
 int launch_zplane_density(const double2* R, double* rho_part, int groups,
                           const double* occ, int64_t state0, int nstates,
                           double inv_vol, const BoxGeom& geo,
-                          const double2* tw, hipStream_t s) {
+                          const double2* tw, bool store, hipStream_t s) {
   if (!use_rowpath(geo)) {
     set_error("row path: unsupported box");
     return JRX_ERR_UNSUPPORTED;
   }
   if (geo.nx == 128)
     return zdens_n<128, 1>(R, rho_part, groups, occ, state0, nstates, inv_vol,
-                           geo, tw, s);
+                           geo, tw, store, s);
   if (geo.nxa <= ZCfg<64>::BLOCK / 8)
     return zdens_n<64, 1>(R, rho_part, groups, occ, state0, nstates, inv_vol,
-                          geo, tw, s);
+                          geo, tw, store, s);
   // more active columns than a 256-thread block has lane groups: a
   // 512-thread block (one column pass; a second pass of 256-thread blocks
   // doubled the registers and halved the occupancy)
   return zdens_n<64, 1, 512>(R, rho_part, groups, occ, state0, nstates,
-                             inv_vol, geo, tw, s);
+                             inv_vol, geo, tw, store, s);
 }
 
 template <int N, int CP, int BLK = ZCfg<N>::BLOCK>

@@ -1,7 +1,8 @@
 """The row-buffer FFT path (cubic 64^3 and 128^3 boxes) against the oracle and
 against the plane-buffer path it replaced (fft_variant 2): energies,
-density, gradient, band energies, two spin channels, chunked sweeps, and a
-128^3 Gamma-point cell of the 64-atom supercell.
+density, gradient, band energies, two spin channels, chunked sweeps, state
+groups left empty by the split of the states, and a 128^3 Gamma-point cell
+of the 64-atom supercell.
 
 Tolerances as the other parity tests: total energy 1e-8 Ha, density
 max-abs 1e-10, gradient 1e-7 relative to its largest entry; the two HIP
@@ -91,6 +92,24 @@ def test_row_path_two_spins_and_chunks():
   assert chunked.row_path and not chunked.keep_planes
   _compare(full, _run(chunked, p))
   _check_oracle(full, wl2, p, spin_restricted=False)
+
+
+def test_row_path_empty_state_groups_repeated():
+  """33 states at one k-point: 32 z-plane state groups of 2 states leave
+  groups 17-31 empty, whose density planes the pass writes as zeros.  Two
+  evaluations with different inputs (the second overwrites the first's
+  partial planes) against the plane path and the oracle."""
+  from jrystal_amd import workloads as W
+  wl = W.build("si8-64^3-gamma-33b", W.diamond_cube(4.0), (64,) * 3,
+               (1, 1, 1), 33, 30.0)
+  row = _plan(wl)
+  plane = _plan(wl, fft_variant=2)
+  assert row.row_path and not plane.row_path
+  for seed in (3, 4):
+    p = seeded_params(wl.shape, seed)
+    a = _run(row, p)
+    _compare(a, _run(plane, p))
+  _check_oracle(a, wl, p)
 
 
 def test_row_path_128_gamma_si64():
