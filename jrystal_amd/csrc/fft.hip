@@ -72,6 +72,14 @@ int launch_line_fft(const double2* in, double2* out, int64_t B, int n0, int n1,
     nouter = B; os = (int64_t)n0 * n1 * n2; Q = (int64_t)n1 * n2; qs = 1;
     es = (int64_t)n1 * n2; elem_major = true;
   }
+  // Lines per block: few enough that one box spreads over about 1024
+  // blocks, and on the strided axes at least 8 adjacent lines (128 bytes
+  // of each element row); large batches keep the LDS-filling kLineCap / n.
+  const int64_t lines = nouter * Q;
+  int want = lines / 1024 > 1 ? (int)(lines / 1024 < LB ? lines / 1024 : LB)
+                              : 1;
+  if (elem_major && want < 8) want = 8;
+  if (want < LB) LB = want;
   if (Q < LB) LB = (int)Q;
   const int64_t qblocks = (Q + LB - 1) / LB;
   const int64_t nblocks = nouter * qblocks;
