@@ -73,12 +73,13 @@ int launch_line_fft(const double2* in, double2* out, int64_t B, int n0, int n1,
     es = (int64_t)n1 * n2; elem_major = true;
   }
   // Lines per block: few enough that one box spreads over about 1024
-  // blocks, and on the strided axes at least 8 adjacent lines (128 bytes
-  // of each element row); large batches keep the LDS-filling kLineCap / n.
+  // blocks, but at least 8 (on the strided axes 128 bytes of each element
+  // row; one line per block measured slower on the contiguous axis too);
+  // large batches keep the LDS-filling kLineCap / n.
   const int64_t lines = nouter * Q;
   int want = lines / 1024 > 1 ? (int)(lines / 1024 < LB ? lines / 1024 : LB)
                               : 1;
-  if (elem_major && want < 8) want = 8;
+  if (want < 8) want = 8;
   if (want < LB) LB = want;
   if (Q < LB) LB = (int)Q;
   const int64_t qblocks = (Q + LB - 1) / LB;
@@ -397,14 +398,23 @@ int launch_xline_vpsi(double2* T, const double* v, int groups, int nstates,
   return JRX_OK;
 }
 
+// out[i] = sum_g part[g][i], four sums in flight (groups g = 4m + u go to
+// sum u), added in a fixed order.
 __global__ void reduce_groups_kernel(const double* __restrict__ part,
                                      double* __restrict__ out, int G,
                                      int64_t N) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double a = 0.0;
-  for (int g = 0; g < G; ++g) a += part[(int64_t)g * N + i];
-  out[i] = a;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int g = 0;
+  for (; g + 3 < G; g += 4) {
+    a0 += part[(int64_t)g * N + i];
+    a1 += part[(int64_t)(g + 1) * N + i];
+    a2 += part[(int64_t)(g + 2) * N + i];
+    a3 += part[(int64_t)(g + 3) * N + i];
+  }
+  for (; g < G; ++g) a0 += part[(int64_t)g * N + i];
+  out[i] = (a0 + a1) + (a2 + a3);
 }
 
 int launch_reduce_groups(const double* part, double* out, int G, int64_t N,
