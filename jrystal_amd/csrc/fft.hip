@@ -234,7 +234,7 @@ template <int NX>
 __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
     const double2* __restrict__ T, double* __restrict__ rho_part,
     const double* __restrict__ occ, int64_t state0, int nstates,
-    double inv_vol, BoxGeom geo, FftLen Lx, int LZ) {
+    double inv_vol, BoxGeom geo, FftLen Lx, int LZ, int store) {
   extern __shared__ double2 sm[];
   const int nx = NX > 0 ? NX : geo.nx;
   const int ny = geo.ny, nz = geo.nz, P = ny * nz;
@@ -245,8 +245,16 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
   const int G = gridDim.y, gi = blockIdx.y;
   const int per = (nstates + G - 1) / G;
   const int cb = gi * per, ce = min(nstates, cb + per);
-  if (cb >= ce) return;  // uniform over the block
   const int nxl = nx * LZ;
+  double* part = rho_part + (int64_t)gi * geo.nx * P;
+  if (cb >= ce) {  // uniform over the block: an empty group
+    if (store)
+      for (int e = threadIdx.x; e < nxl; e += kXBlock) {
+        const int x = e / LZ, l = e % LZ;
+        if (l < nl) part[((int64_t)x * ny + y) * nz + z0 + l] = 0.0;
+      }
+    return;
+  }
   int goff[kXEpt];
   xline_slots(goff, geo, nxl, LZ, nl);
   const double2* src0 = T + (int64_t)y * nz + z0;
@@ -276,14 +284,14 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
     }
     __syncthreads();
   }
-  double* part = rho_part + (int64_t)gi * geo.nx * P;
+  // the group's partial: stored by the first sweep chunk, added to after
 #pragma unroll
   for (int k = 0; k < kXEpt; ++k) {
     const int e = threadIdx.x + k * kXBlock;
     const int x = e / LZ, l = e % LZ;
     if (e < nxl && l < nl) {
       const int64_t idx = ((int64_t)x * ny + y) * nz + z0 + l;
-      part[idx] += acc[k] * inv_vol;
+      part[idx] = store ? acc[k] * inv_vol : part[idx] + acc[k] * inv_vol;
     }
   }
 }
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(kXBlock) void xline_density_kernel(
 int launch_xline_density(const double2* T, double* rho_part, int groups,
                          const double* occ, int64_t state0, int nstates,
                          double inv_vol, const BoxGeom& geo, const FftLen& Lx,
-                         hipStream_t s) {
+                         bool store, hipStream_t s) {
   if (geo.nx > kXCap) {
     set_error("x grid size above 2048 is not supported");
     return JRX_ERR_UNSUPPORTED;
@@ -304,14 +312,14 @@ int launch_xline_density(const double2* T, double* rho_part, int groups,
   case NN:                                                                  \
     hipLaunchKernelGGL((xline_density_kernel<NN>), grid, block, shm, s, T,  \
                        rho_part, occ, state0, nstates, inv_vol, geo, Lx,    \
-                       LZ);                                                 \
+                       LZ, store ? 1 : 0);                                  \
     break;
   switch (geo.nx) {
     JRX_CT_SIZES(JRX_XD)
     default:
       hipLaunchKernelGGL((xline_density_kernel<0>), grid, block, shm, s, T,
                          rho_part, occ, state0, nstates, inv_vol, geo, Lx,
-                         LZ);
+                         LZ, store ? 1 : 0);
   }
 #undef JRX_XD
   JRX_LAUNCH_CHECK();
@@ -683,7 +691,8 @@ __global__ __launch_bounds__(kPlaneBlock, 4) void plane_fft_gather64_kernel(
 __global__ __launch_bounds__(kXBlock) void xline_density64_kernel(
     const double2* __restrict__ T, double* __restrict__ rho_part,
     const double* __restrict__ occ, int64_t state0, int nstates,
-    double inv_vol, BoxGeom geo, const double2* __restrict__ tw64) {
+    double inv_vol, BoxGeom geo, const double2* __restrict__ tw64,
+    int store) {
   __shared__ double2 ex[(kXBlock / 8) * kExLine];
   __shared__ double2 stw[64];
   const int tid = threadIdx.x, j = tid & 7, line = tid >> 3;
@@ -695,9 +704,15 @@ __global__ __launch_bounds__(kXBlock) void xline_density64_kernel(
   const int G = gridDim.y, gi = blockIdx.y;
   const int per = (nstates + G - 1) / G;
   const int cb = gi * per, ce = min(nstates, cb + per);
+  double* part = rho_part + (int64_t)gi * 64 * P + (int64_t)y * nz + z;
   if (tid < 64) stw[tid] = tw64[tid];
   __syncthreads();
-  if (cb >= ce) return;
+  if (cb >= ce) {  // uniform over the block: an empty group
+    if (store && zok)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) part[(int64_t)(j + 8 * t) * P] = 0.0;
+    return;
+  }
   int xo[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -730,9 +745,12 @@ __global__ __launch_bounds__(kXBlock) void xline_density64_kernel(
     for (int t = 0; t < 8; ++t) acc[t] += f * (v[t].x * v[t].x + v[t].y * v[t].y);
   }
   if (!zok) return;
-  double* part = rho_part + (int64_t)gi * 64 * P + (int64_t)y * nz + z;
+  // the group's partial: stored by the first sweep chunk, added to after
 #pragma unroll
-  for (int t = 0; t < 8; ++t) part[(int64_t)(j + 8 * t) * P] += acc[t] * inv_vol;
+  for (int t = 0; t < 8; ++t) {
+    double* dst = part + (int64_t)(j + 8 * t) * P;
+    *dst = store ? acc[t] * inv_vol : *dst + acc[t] * inv_vol;
+  }
 }
 
 __global__ __launch_bounds__(kXBlock) void xline_vpsi64_kernel(
@@ -827,11 +845,11 @@ int launch_plane_fft_gather64(const double2* T, const double2* q,
 int launch_xline_density64(const double2* T, double* rho_part, int groups,
                            const double* occ, int64_t state0, int nstates,
                            double inv_vol, const BoxGeom& geo,
-                           const double2* tw64, hipStream_t s) {
+                           const double2* tw64, bool store, hipStream_t s) {
   const int nzt = (geo.nz + 31) / 32;
   hipLaunchKernelGGL(xline_density64_kernel, dim3(geo.ny * nzt, groups),
                      dim3(kXBlock), 0, s, T, rho_part, occ, state0, nstates,
-                     inv_vol, geo, tw64);
+                     inv_vol, geo, tw64, store ? 1 : 0);
   JRX_LAUNCH_CHECK();
   return JRX_OK;
 }

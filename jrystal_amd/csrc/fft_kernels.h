@@ -57,11 +57,12 @@ int xlines_per_block(int nx);
 int xline_groups(const BoxGeom& geo, int nstates, int target_blocks = 0);
 
 // rho_part[g](x,y,z) += sum_{c in group g} occ[state0+c] |IFFT_x T[c]|^2 /
-// vol.  All states of the chunk belong to one spin.
+// vol, or = with store (then every group's partial is written, empty groups
+// with zeros).  All states of the chunk belong to one spin.
 int launch_xline_density(const double2* T, double* rho_part, int groups,
                          const double* occ, int64_t state0, int nstates,
                          double inv_vol, const BoxGeom& geo, const FftLen& Lx,
-                         hipStream_t s);
+                         bool store, hipStream_t s);
 
 // out[i] = sum_g part[g*N + i] (fixed order).
 int launch_reduce_groups(const double* part, double* out, int G, int64_t N,
@@ -105,7 +106,7 @@ int launch_plane_fft_gather64(const double2* T, const double2* q,
 int launch_xline_density64(const double2* T, double* rho_part, int groups,
                            const double* occ, int64_t state0, int nstates,
                            double inv_vol, const BoxGeom& geo,
-                           const double2* tw64, hipStream_t s);
+                           const double2* tw64, bool store, hipStream_t s);
 int launch_xline_vpsi64(double2* T, const double* v, int groups, int nstates,
                         const BoxGeom& geo, const double2* tw64,
                         hipStream_t s);

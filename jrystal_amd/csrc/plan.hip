@@ -1236,11 +1236,13 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
   if (!fixed) {
     for (int sp = 0; sp < p->ns; ++sp) {
       const int64_t step = p->keep_planes ? SS : p->chunk;
-      // on the row path the first chunk stores every group's plane when it
-      // runs all p->groups groups; otherwise the partials start from zero
+      // the first chunk stores every group's partial when it runs all
+      // p->groups groups; otherwise the partials start from zero
+      const int first = (int)(SS < step ? SS : step);
       const bool store_first =
-          p->rowpath &&
-          zplane_groups(p->geo, (int)(SS < step ? SS : step)) == p->groups;
+          (p->rowpath ? zplane_groups(p->geo, first)
+                      : xline_groups(p->geo, first, p->xline_blocks)) ==
+          p->groups;
       if (!store_first)
         JRX_T(JRX_K_OTHER, launch_fill(rpart, 0.0, (int64_t)p->groups * p->N, s));
       for (int64_t s0 = 0; s0 < SS; s0 += step) {
@@ -1261,14 +1263,15 @@ static int energy_grad_impl(jrx_plan* p, const double* w_re,
           JRX_T(JRX_K_XLINE_DENSITY,
                 launch_xline_density64(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
                                        p->d_occ, st0, nst, 1.0 / p->vol,
-                                       p->geo, p->Lx.tw, s));
+                                       p->geo, p->Lx.tw,
+                                       store_first && s0 == 0, s));
         } else {
           JRX_T(JRX_K_PLANE_IFFT,
                 launch_plane_ifft(q, Tc, st0, nst, p->geo, p->Ly, p->Lz, s));
           JRX_T(JRX_K_XLINE_DENSITY,
                 launch_xline_density(Tc, rpart, xline_groups(p->geo, nst, p->xline_blocks),
                                      p->d_occ, st0, nst, 1.0 / p->vol, p->geo,
-                                     p->Lx, s));
+                                     p->Lx, store_first && s0 == 0, s));
         }
       }
       JRX_T(JRX_K_XLINE_DENSITY,
