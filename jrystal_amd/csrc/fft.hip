@@ -20,10 +20,13 @@ constexpr int kLineCap = 2048;
 // Lines (o, q): base = o*os + q*qs, element stride es.  A block transforms
 // LB consecutive q of one o.  ELEM_MAJOR keeps LDS as [j][LB] (lines
 // adjacent, for strided axes); otherwise [l][n+1] (for contiguous lines).
-template <bool INV, bool ELEM_MAJOR>
+// With REAL the input is instead the real box sum_s rin[s][.] (ns spin
+// channels of rn points each, added in spin order).
+template <bool INV, bool ELEM_MAJOR, bool REAL = false>
 __global__ __launch_bounds__(kLineBlock) void line_fft_kernel(
     const double2* __restrict__ in, double2* __restrict__ out, FftLen L,
-    int64_t os, int64_t Q, int64_t qs, int64_t es, int LB, double scale) {
+    int64_t os, int64_t Q, int64_t qs, int64_t es, int LB, double scale,
+    const double* __restrict__ rin = nullptr, int ns = 0, int64_t rn = 0) {
   extern __shared__ double2 sm[];
   const int n = L.n;
   const int64_t qblocks = (Q + LB - 1) / LB;
@@ -37,7 +40,15 @@ __global__ __launch_bounds__(kLineBlock) void line_fft_kernel(
   for (int e = threadIdx.x; e < tot; e += kLineBlock) {
     int l, j;
     if (ELEM_MAJOR) { l = e % nl; j = e / nl; } else { j = e % n; l = e / n; }
-    sm[ELEM_MAJOR ? j * LB + l : l * (n + 1) + j] = src[l * qs + j * es];
+    double2 x;
+    if constexpr (REAL) {
+      double r = 0.0;
+      for (int c = 0; c < ns; ++c) r += rin[c * rn + off + l * qs + j * es];
+      x = make_double2(r, 0.0);
+    } else {
+      x = src[l * qs + j * es];
+    }
+    sm[ELEM_MAJOR ? j * LB + l : l * (n + 1) + j] = x;
   }
   __syncthreads();
   if (ELEM_MAJOR)
@@ -54,7 +65,12 @@ __global__ __launch_bounds__(kLineBlock) void line_fft_kernel(
 
 int launch_line_fft(const double2* in, double2* out, int64_t B, int n0, int n1,
                     int n2, int axis, bool inverse, double scale,
-                    const FftLen& L, hipStream_t s) {
+                    const FftLen& L, hipStream_t s, const double* rin,
+                    int ns) {
+  if (rin && (axis != 2 || inverse || B != 1)) {
+    set_error("launch_line_fft: a real input is for one forward z pass");
+    return JRX_ERR_INVALID;
+  }
   const int n = L.n;
   if (n > kLineCap) {
     set_error("FFT length above 2048 is not supported");
@@ -92,6 +108,13 @@ int launch_line_fft(const double2* in, double2* out, int64_t B, int n0, int n1,
   size_t shm = elem_major ? (size_t)n * LB * sizeof(double2)
                           : (size_t)LB * (n + 1) * sizeof(double2);
   dim3 grid((unsigned)nblocks), block(kLineBlock);
+  if (rin) {
+    hipLaunchKernelGGL((line_fft_kernel<false, false, true>), grid, block,
+                       shm, s, nullptr, out, L, os, Q, qs, es, LB, scale, rin,
+                       ns, (int64_t)n0 * n1 * n2);
+    JRX_LAUNCH_CHECK();
+    return JRX_OK;
+  }
   if (inverse) {
     if (elem_major)
       hipLaunchKernelGGL((line_fft_kernel<true, true>), grid, block, shm, s,
@@ -119,6 +142,17 @@ int fft3d(const double2* in, double2* out, int64_t B, int n0, int n1, int n2,
   rc = launch_line_fft(out, out, B, n0, n1, n2, 1, inverse, 1.0, Ly, s);
   if (rc) return rc;
   return launch_line_fft(out, out, B, n0, n1, n2, 0, inverse, scale, Lx, s);
+}
+
+int fft3d_real_forward(const double* rin, int ns, double2* out, int n0,
+                       int n1, int n2, const FftLen& Lx, const FftLen& Ly,
+                       const FftLen& Lz, hipStream_t s) {
+  int rc = launch_line_fft(nullptr, out, 1, n0, n1, n2, 2, false, 1.0, Lz, s,
+                           rin, ns);
+  if (rc) return rc;
+  rc = launch_line_fft(out, out, 1, n0, n1, n2, 1, false, 1.0, Ly, s);
+  if (rc) return rc;
+  return launch_line_fft(out, out, 1, n0, n1, n2, 0, false, 1.0, Lx, s);
 }
 
 // ------------------------------------------------------- fused, pruned --

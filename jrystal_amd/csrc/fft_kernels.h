@@ -34,14 +34,22 @@ struct BoxGeom {
 };
 
 // y = scale * DFT_axis(x) along one axis of a (B, n0, n1, n2) complex box.
+// rin (forward axis-2 pass of one box only): x = sum_s rin[s] of ns real
+// boxes instead of in.
 int launch_line_fft(const double2* in, double2* out, int64_t B, int n0, int n1,
                     int n2, int axis, bool inverse, double scale,
-                    const FftLen& L, hipStream_t s);
+                    const FftLen& L, hipStream_t s,
+                    const double* rin = nullptr, int ns = 0);
 
 // Full 3D transform (three line passes).  In-place allowed.
 int fft3d(const double2* in, double2* out, int64_t B, int n0, int n1, int n2,
           bool inverse, double scale, const FftLen& Lx, const FftLen& Ly,
           const FftLen& Lz, hipStream_t s);
+// Forward 3D transform of the real box sum_s rin[s] (ns boxes of
+// n0 n1 n2 points, added in spin order) into the complex box out.
+int fft3d_real_forward(const double* rin, int ns, double2* out, int n0,
+                       int n1, int n2, const FftLen& Lx, const FftLen& Ly,
+                       const FftLen& Lz, hipStream_t s);
 
 // T[c][xi][y][z] = 2D inverse transform over (y,z) of the scattered plane
 // x = plane_x[xi] of state (state0 + c); q is band-major [state][ng].

@@ -1120,9 +1120,8 @@ static int potential_stage(jrx_plan* p, double* rho, double* v, char* ws,
   const CellParams& cpb = p->rowpath ? p->cpz : p->cp;
   const FftLen& L0 = p->rowpath ? p->Lz : p->Lx;
   const FftLen& L2 = p->rowpath ? p->Lx : p->Lz;
-  JRX_T(JRX_K_POTENTIAL, launch_rho_to_complex(rho, box, p->ns, p->N, s));
-  JRX_T(JRX_K_POTENTIAL, fft3d(box, box, 1, cpb.nx, cpb.ny, cpb.nz, false,
-                               1.0, L0, p->Ly, L2, s));
+  JRX_T(JRX_K_POTENTIAL, fft3d_real_forward(rho, p->ns, box, cpb.nx, cpb.ny,
+                                            cpb.nz, L0, p->Ly, L2, s));
   double2* gb = p->o_gb ? cptr(p->o_gb) : nullptr;
   if (gb)  // density gradients of the GGA terms, from fftn(rho)
     JRX_T(JRX_K_POTENTIAL,
