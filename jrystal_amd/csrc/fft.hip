@@ -245,14 +245,18 @@ __device__ __forceinline__ void xline_load(double2 (&pre)[kXEpt],
     pre[k] = goff[k] >= 0 ? nt_load(src + goff[k]) : czero();
 }
 
-int xlines_per_block(int nx) {
+int xlines_per_block(int nx, int nz) {
   int LZ = kXCap / nx;
   if (LZ > kXLines) LZ = kXLines;
-  return LZ < 1 ? 1 : LZ;
+  if (LZ < 1) return 1;
+  // as few z-tiles as this allows, of equal size (48 lines: 2 x 24, not
+  // 32 + 16)
+  const int tiles = (nz + LZ - 1) / LZ;
+  return (nz + tiles - 1) / tiles;
 }
 
 int xline_groups(const BoxGeom& geo, int nstates, int target_blocks) {
-  const int LZ = xlines_per_block(geo.nx);
+  const int LZ = xlines_per_block(geo.nx, geo.nz);
   const int tiles = geo.ny * ((geo.nz + LZ - 1) / LZ);
   const int target = target_blocks > 0 ? target_blocks : 4096;
   int G = (target + tiles - 1) / tiles;
@@ -338,7 +342,7 @@ int launch_xline_density(const double2* T, double* rho_part, int groups,
     set_error("x grid size above 2048 is not supported");
     return JRX_ERR_UNSUPPORTED;
   }
-  const int LZ = xlines_per_block(geo.nx);
+  const int LZ = xlines_per_block(geo.nx, geo.nz);
   const int nzt = (geo.nz + LZ - 1) / LZ;
   size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
   dim3 grid(geo.ny * nzt, groups), block(kXBlock);
@@ -420,7 +424,7 @@ int launch_xline_vpsi(double2* T, const double* v, int groups, int nstates,
     set_error("x grid size above 2048 is not supported");
     return JRX_ERR_UNSUPPORTED;
   }
-  const int LZ = xlines_per_block(geo.nx);
+  const int LZ = xlines_per_block(geo.nx, geo.nz);
   const int nzt = (geo.nz + LZ - 1) / LZ;
   size_t shm = (size_t)geo.nx * LZ * sizeof(double2) + (size_t)geo.nxa * 4 + 16;
   dim3 grid(geo.ny * nzt, groups), block(kXBlock);

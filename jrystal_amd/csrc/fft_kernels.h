@@ -59,7 +59,7 @@ int launch_plane_ifft(const double2* q, double2* T, int64_t state0,
 
 // Groups of states per x-line launch (parallelism across states).
 constexpr int kMaxXGroups = 32;
-int xlines_per_block(int nx);
+int xlines_per_block(int nx, int nz);
 // State groups of an x-line pass: enough groups to reach about
 // target_blocks workgroups (0 = 4096), at most kMaxXGroups and nstates.
 int xline_groups(const BoxGeom& geo, int nstates, int target_blocks = 0);
