@@ -540,8 +540,14 @@ int row_chunks(const BoxGeom& geo) {
 }
 
 int zplane_groups(const BoxGeom& geo, int nstates) {
-  // enough blocks (z-planes x groups) to fill the device a few times over
-  int G = (2048 + geo.nz - 1) / geo.nz;
+  // about 2048 blocks (z-planes x groups) when every group gets 16 states
+  // or more; with fewer states, fewer groups down to about 512 blocks, each
+  // group then taking several states (one state per block repeats the
+  // block's set-up for little work)
+  const int hi = (2048 + geo.nz - 1) / geo.nz, lo = (512 + geo.nz - 1) / geo.nz;
+  int G = nstates / 16;
+  if (G > hi) G = hi;
+  if (G < lo) G = lo;
   if (G > nstates) G = nstates;
   return G < 1 ? 1 : G;
 }
